@@ -1,0 +1,108 @@
+/*
+ * umcx.h -- C ABI of the MI355X umcx photon-transport library (libumcx.so).
+ *
+ * Drop-in part: the two C exports of fangq/umcx (src/umcx.cpp:850-856), same names, arguments
+ * and outputs (files <ID>.bnii / <ID>_detp.jdb in the working directory plus one summary line
+ * on stdout). C++ callers also get MCX_run_simulation with the reference's signature
+ * (src/umcx.cpp:783). Unlike the reference, no C++ exception crosses this boundary: every entry
+ * point returns 0 on success and non-zero on failure, with the message in umcx_last_error().
+ *
+ * Session part: the photon loop MCX_kernel (src/umcx.cpp:722-781) as a reusable GPU session,
+ * so a caller can prepare a problem once, run photon ranges on the GPU, and read tallies,
+ * detected photons and energies back without going through files.
+ */
+#ifndef UMCX_H
+#define UMCX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- drop-in exports ---------------------------------------------------------------- */
+
+/* Replaces MCX_run_cmd (src/umcx.cpp:850): argv/argn as for the umcx command line. */
+int MCX_run_cmd(char* argv[], int argn);
+/* Replaces MCX_run_json (src/umcx.cpp:853): one JSON document merged into an empty config. */
+int MCX_run_json(char* jsoninput);
+
+/* Message of the last failure on this thread ("" when none). */
+const char* umcx_last_error(void);
+const char* umcx_version(void);
+int umcx_device_count(void);
+
+/* ---- GPU session ---------------------------------------------------------------------- */
+
+typedef struct umcx_session umcx_session;
+
+/* Launch options; zero fields take tuned defaults. No counterpart in the reference, whose
+ * ThreadNum/BlockSize session keys (src/umcx.cpp:739-741) are honoured when these are zero. */
+typedef struct umcx_launch {
+    int32_t device;      /* HIP device ordinal */
+    int32_t block;       /* threads per workgroup */
+    int32_t blocks;      /* persistent workgroups */
+    int32_t tally_mode;  /* -1 tuned, 0 global atomics, 1 LDS tile */
+    int32_t trace;       /* 1: keep every photon's final state (small runs) */
+} umcx_launch;
+
+/* Constants of a prepared problem: MCX_param (src/umcx.cpp:95-99) plus launch facts. */
+typedef struct umcx_info {
+    uint32_t dims[4];    /* nx, ny, nz, maxgate */
+    uint64_t nphoton;
+    int32_t mediumnum, detnum, detp_columns, ppath_columns;
+    int32_t outputtype, isnormalized, issavevol, issavedet, isreflect, srctype;
+    float tstart, tend, rtstep, unitinmm;
+    uint32_t seeds[4];
+    int32_t block, blocks, tally_mode, label_bytes;
+} umcx_info;
+
+/* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)
+ * or from command-line arguments. Uploads the domain to the GPU. */
+int umcx_session_from_json(const char* json, const umcx_launch* launch, umcx_session** out);
+int umcx_session_from_argv(char** argv, int argn, const umcx_launch* launch, umcx_session** out);
+int umcx_session_info(const umcx_session* s, umcx_info* info);
+/* Zero the fluence grid, detected photons and energy tallies. */
+int umcx_session_reset(umcx_session* s);
+/* Run photons [first, first+count) of the MCX_kernel loop (src/umcx.cpp:752-775) and wait;
+ * tallies accumulate. kernel_ms (may be NULL) receives the HIP-event time of the kernel. */
+int umcx_session_run(umcx_session* s, uint64_t first, uint64_t count, double* kernel_ms);
+/* Raw (unnormalised) fluence tallies, memory order [t][x][y][z]; n = dims product. */
+int umcx_session_fluence(const umcx_session* s, float* out, uint64_t n);
+/* Device pointer of the fluence grid (stays owned by the session). */
+int umcx_session_fluence_device(const umcx_session* s, float** dptr);
+/* All detections (incl. ones beyond MaxDetPhoton) and the number of rows kept (:182-184). */
+int umcx_session_detected(const umcx_session* s, uint32_t* detected, uint32_t* saved);
+/* Detected-photon rows [saved x detp_columns] in the layout of src/umcx.cpp:164-176, plus the
+ * photon index and final weight of each row (both extensions). Any pointer may be NULL. */
+int umcx_session_detp(const umcx_session* s, float* rows, uint32_t* ids, float* weights);
+/* {escaped, launched, deposited} weight sums in fp64 (src/umcx.cpp:765-770). */
+int umcx_session_energies(const umcx_session* s, double out[3]);
+/* Per-photon final state of the last run (64 bytes each), when launched with trace = 1. */
+int umcx_session_trace(const umcx_session* s, void* out, uint64_t n);
+/* Output normaliser of src/umcx.cpp:820 for a given launched energy. */
+float umcx_session_normalizer(const umcx_session* s, double energytotal);
+void umcx_session_free(umcx_session* s);
+
+/* ---- host-only helpers (no GPU needed) ---------------------------------------------------- */
+
+/* Label volume with detector bits, as MCX_userio::initdomain + maskdetectors build it
+ * (src/umcx.cpp:537-632). *labels is allocated by the library; release with umcx_free. */
+int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels);
+/* Expanded config after flags/presets/merges, as --dumpjson prints it (src/umcx.cpp:525-527). */
+int umcx_config_from_argv(char** argv, int argn, char** json_out);
+/* glibc srand(seed)+4*rand() words used to seed every photon (src/umcx.cpp:725-727). */
+int umcx_seed_words(int32_t seed, uint32_t out[4]);
+/* Evaluate the libm replicas (fn 0 logf, 1 sinf, 2 cosf, 3 acosf) on the host CPU. */
+int umcx_libm_host(int32_t fn, const float* in, float* out, uint64_t n);
+/* Same on the GPU (device ordinal), for bitwise checks of the device build. */
+int umcx_libm_device(int32_t fn, const float* in, float* out, uint64_t n, int32_t device);
+void umcx_free(void* p);
+
+#ifdef __cplusplus
+}
+/* C++ linkage, as in the reference (src/umcx.cpp:783). argn < 0 (a BJData buffer from MATLAB)
+ * is rejected with an error. */
+int MCX_run_simulation(char* argv[], int argn = 1, int nlhs = 0, void* plhs[] = nullptr);
+#endif
+
+#endif /* UMCX_H */

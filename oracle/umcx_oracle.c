@@ -43,6 +43,14 @@ float oracle_host_cosf(float x) { return cosf(x); }
 float oracle_host_acosf(float x) { return acosf(x); }
 float oracle_host_expf(float x) { return expf(x); }
 
+void oracle_host_libm_array(int fn, const float* in, float* out, uint64_t n) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        float x = in[i];
+        out[i] = fn == 0 ? logf(x) : fn == 1 ? sinf(x) : fn == 2 ? cosf(x) : fn == 3 ? acosf(x) : expf(x);
+    }
+}
+
 /* glibc srandom/random TYPE_3: r[0]=seed, r[i]=16807*r[i-1] mod (2^31-1) for i<31,
  * r[31..33]=r[0..2], r[i]=r[i-31]+r[i-3]; the k-th output is r[k+344]>>1.
  * Restates std::srand(seed) followed by four std::rand() (src/umcx.cpp:725-727). */

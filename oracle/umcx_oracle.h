@@ -53,6 +53,8 @@ float oracle_host_sinf(float x);
 float oracle_host_cosf(float x);
 float oracle_host_acosf(float x);
 float oracle_host_expf(float x);
+/* fn: 0 logf, 1 sinf, 2 cosf, 3 acosf, 4 expf of the host C library over an array. */
+void oracle_host_libm_array(int fn, const float* in, float* out, uint64_t n);
 
 /*
  * Run photons [first, first+count) of the MCX_kernel loop (:752-775).

@@ -153,7 +153,11 @@ def build_case(name, argv, cfg):
         argv = list(argv) + ["-s", name]
         dump = run_ref(argv + ["--dumpjson"], td)
         run_ref(argv + ["--dumpmask"], td)
-        mask = bjdata.load_bnii(os.path.join(td, f"{name}_vol.bnii"))[..., 0].astype(np.int32)
+        maskfile = os.path.join(td, f"{name}_vol.bnii")
+        mask = bjdata.load_bnii(maskfile)[..., 0].astype(np.int32)
+        with open(maskfile, "rb") as f:
+            mask_head = np.frombuffer(f.read(128), dtype=np.uint8)
+        mask_size = os.path.getsize(maskfile)
         out = run_ref(argv, td)
         m = LINE.search(out)
         if not m:
@@ -162,6 +166,8 @@ def build_case(name, argv, cfg):
                    energy=float(m.group(1)), normalizer=float(m.group(4)), detected=int(m.group(5)),
                    absorbed=float(m.group(6)) if "nan" not in m.group(6) else np.nan, line=out.strip())
         rec["mask"] = mask
+        rec["mask_head"] = mask_head
+        rec["mask_size"] = mask_size
         bnii = os.path.join(td, f"{name}.bnii")
         if os.path.exists(bnii):
             flu = bjdata.load_bnii(bnii)          # [x,y,z,t]

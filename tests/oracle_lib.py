@@ -61,11 +61,41 @@ def lib():
         _lib.oracle_run.restype = C.c_int
         _lib.oracle_seed_words.argtypes = [C.c_int32, C.POINTER(C.c_uint32)]
         _lib.oracle_detp_len.argtypes = [C.POINTER(OracleProblem)]
+        _lib.oracle_host_libm_array.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64]
+        _lib.oracle_set_mutant.argtypes = [C.c_int]
         for fn in ("logf", "sinf", "cosf", "acosf", "expf"):
             f = getattr(_lib, "oracle_host_" + fn)
             f.argtypes = [C.c_float]
             f.restype = C.c_float
     return _lib
+
+
+def host_libm(fn: int, x: np.ndarray) -> np.ndarray:
+    """The host C library's logf/sinf/cosf/acosf/expf (fn 0..4) over an array."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    lib().oracle_host_libm_array(fn, x.ctypes.data, out.ctypes.data, x.size)
+    return out
+
+
+def reachable_inputs(g_values=(0.9, 0.01, 0.89, 0.95, 0.7)) -> dict:
+    """Every argument the photon loop can hand to a trajectory-affecting libm call: u on the
+    2^23 grid of rand01 (src/umcx.cpp:209-210); logf(u+eps) (:213); sin/cos(2 pi u) (:366-367);
+    acosf of the Henyey-Greenstein cosine per medium g (:370-372) and of 2u-1 (:374)."""
+    F = np.float32
+    k = np.arange(1 << 23, dtype=np.uint32)
+    u = (k | np.uint32(0x3F800000)).view(np.float32) - F(1)
+    out = {0: [u + F(np.finfo(np.float32).eps)], 1: [], 2: [], 3: []}
+    phi = F(6.2831855) * u
+    out[1].append(phi)
+    out[2].append(phi)
+    out[3].append(F(2) * u - F(1))
+    for g in g_values:
+        g = F(g)
+        t = (F(1) - g * g) / (F(1) - g + F(2) * g * u)
+        t = (F(1) + g * g - t * t) / (F(2) * g)
+        out[3].append(np.maximum(F(-1), np.minimum(F(1), t)))
+    return {fn: np.concatenate(v) for fn, v in out.items()}
 
 
 def seed_words(seed: int) -> list[int]:

@@ -1,0 +1,199 @@
+"""GPU parity: the gfx950 kernel (through libumcx.so's C ABI) against the oracle and the
+reference's golden vectors.
+
+Bars:
+* per photon: position, direction, scatter count, time of flight, path and medium bitwise equal
+  to the oracle; the weight within 1e-5 relative (it decays through expf, whose last bit may
+  differ between the device and glibc; weights never steer photons);
+* detected-photon rows: bitwise equal to the reference's rows as a multiset;
+* fluence: relative L2 <= 1e-4 against the reference's volume;
+* energy balance from fp64 tallies: |deposited + escaped - launched| <= 1e-6 launched;
+* absorbed fraction within 1e-3 percentage points of the reference's printed value.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import umcx_amd as U
+from umcx_amd import bjdata
+
+pytestmark = pytest.mark.gpu
+
+TRACE_CASES = ["c1", "cube60b", "c2", "c3", "isotropic", "cone", "cube60planar", "fourier", "sphshells",
+               "spherebox", "zipvol", "c4src", "skinvessel", "gated"]
+DETP_CASES = ["cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "c2", "c4src", "zipvol",
+              "fourier", "save2pt0", "gated"]
+FLUENCE_CASES = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel",
+                 "c2", "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "gated", "normx"]
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def sorted_rows(rows):
+    return rows[np.lexsort(rows.T[::-1])] if rows.size else rows
+
+
+def test_device_libm_bitwise_on_reachable_inputs():
+    say("device libm replicas vs host libm")
+    for fn, x in O.reachable_inputs().items():
+        dev = U.libm_eval(fn, x, device=0)
+        ref = O.host_libm(fn, x)
+        assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32)), f"fn {fn}"
+
+
+@pytest.mark.parametrize("name", TRACE_CASES)
+def test_per_photon_state_matches_oracle(name):
+    g, prob = O.golden_problem(name)
+    n = 3000 if name not in ("c3", "skinvessel") else 1500
+    say(f"trace parity {name} n={n}")
+    o = O.run(prob, nphoton=n, traces=True, nthreads=8)["traces"]
+    with U.Simulation(g["cfg"], trace=True) as sim:
+        sim.run(0, n)
+        t = sim.trace(n)
+    exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
+    assert np.array_equal(exact(t["pos"][:, :3]), exact(o["pos"][:, :3]))
+    assert np.array_equal(exact(t["vec"]), exact(o["vec"]))
+    assert np.array_equal(exact(t["len"][:, :3]), exact(o["len"][:, :3]))
+    assert np.array_equal(t["mediaid"], o["mediaid"])
+    assert np.array_equal(t["lastvoxelidx"], o["lastvoxelidx"])
+    assert np.array_equal(exact(t["w0"]), exact(o["w0"]))
+    np.testing.assert_allclose(t["pos"][:, 3], o["pos"][:, 3], rtol=1e-5, atol=1e-37)
+    np.testing.assert_allclose(t["len"][:, 3], o["len"][:, 3], rtol=1e-5, atol=1e-37)
+
+
+@pytest.mark.parametrize("name", DETP_CASES)
+def test_detected_rows_match_reference(name):
+    g = O.golden(name)
+    say(f"detp parity {name}")
+    with U.Simulation(g["cfg"]) as sim:
+        sim.run()
+        rows, ids, w = sim.detp()
+        detected, saved = sim.detected()
+    assert saved == int(g["detected"])
+    got = sorted_rows(rows)
+    assert got.shape == g["detp"].shape
+    assert np.array_equal(got.view(np.uint32), g["detp"].view(np.uint32))
+    assert len(np.unique(ids)) >= 1 and ids.max() < int(float(g["cfg"]["Session"]["Photons"]))
+
+
+def test_detector_overflow_keeps_genuine_rows():
+    g, prob = O.golden_problem("overflow")
+    with U.Simulation(g["cfg"]) as sim:
+        sim.run()
+        rows, ids, w = sim.detp()
+        detected, saved = sim.detected()
+    full = O.run(dict(prob, maxdetphotons=10 ** 6), nthreads=8)
+    assert saved == 10 and detected == full["detected"]
+    allrows = {r.tobytes() for r in full["detp"]}
+    assert all(r.tobytes() in allrows for r in rows)
+
+
+@pytest.mark.parametrize("name", FLUENCE_CASES)
+def test_fluence_and_energy_match_reference(name):
+    g, prob = O.golden_problem(name)
+    say(f"fluence parity {name}")
+    with U.Simulation(g["cfg"]) as sim:
+        sim.run()
+        normalize = bool(prob["isnormalized"]) and "-U" not in g["argv"]
+        res = sim.result(normalize=normalize)
+    assert res.energy_total == pytest.approx(float(g["energy"]), abs=0.006)
+    assert abs(res.absorbed_percent - float(g["absorbed"])) <= 1e-3
+    assert abs(res.energy_deposit + res.energy_escape - res.energy_total) <= 1e-6 * res.energy_total
+    f = res.fluence.reshape(-1)
+    if "fluence" in g:
+        ref = g["fluence"]
+        assert np.linalg.norm(f - ref) / np.linalg.norm(ref) <= 1e-4
+    else:
+        idx, ref = g["sample_idx"], g["sample_val"]
+        assert np.linalg.norm(f[idx] - ref) / np.linalg.norm(ref) <= 1e-4
+        ss = float((f.astype(np.float64) ** 2).sum())
+        assert abs(ss - float(g["sumsq"])) / float(g["sumsq"]) <= 2e-4
+    np.testing.assert_allclose(res.fluence.astype(np.float64).sum(axis=(1, 2, 3)), g["gate_sums"], rtol=1e-4,
+                               atol=1e-30)
+
+
+@pytest.mark.parametrize("name", ["cube60", "cube60b", "c2", "c3"])
+def test_command_line_outputs_are_byte_compatible(name, tmp_path):
+    g = O.golden(name)
+    argv = list(g["argv"])
+    if "INPUT.json" in argv:
+        (tmp_path / "INPUT.json").write_text(json.dumps(g["cfg"]))
+    say(f"CLI outputs {name}")
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        assert U.run_cmd(argv) == 0, U.last_error()
+    finally:
+        os.chdir(cwd)
+    bnii = (tmp_path / f"{name}.bnii").read_bytes()
+    assert len(bnii) == int(g["bnii_size"])
+    head = bytes(g["bnii_head"])
+    hlen = bnii.find(b"NIFTIData") + 30       # header up to the raw payload
+    assert bnii[:hlen] == head[:hlen]
+    if "detp" in g:
+        raw = (tmp_path / f"{name}_detp.jdb").read_bytes()
+        assert len(raw) == int(g["detp_size"])
+        dh = bytes(g["detp_head"])
+        assert raw[:len(dh)] == dh
+        info, rows = bjdata.load_detp(str(tmp_path / f"{name}_detp.jdb"))
+        assert np.array_equal(sorted_rows(rows).view(np.uint32), g["detp"].view(np.uint32))
+
+
+def test_photon_ranges_compose():
+    """Photon i depends only on i: two half runs give the same detections as one full run."""
+    g = O.golden("c2")
+    cfg = dict(g["cfg"])
+    n = 400000
+    say("range composition c2")
+    with U.Simulation(cfg) as sim:
+        sim.run(0, n)
+        full_rows = sorted_rows(sim.detp()[0])
+        full_e = sim.energies()
+        sim.reset()
+        sim.run(0, n // 2)
+        sim.run(n // 2, n - n // 2)
+        split_rows = sorted_rows(sim.detp()[0])
+        split_e = sim.energies()
+    assert np.array_equal(full_rows.view(np.uint32), split_rows.view(np.uint32))
+    assert split_e[1] == pytest.approx(full_e[1], rel=1e-12)
+    assert split_e[0] == pytest.approx(full_e[0], rel=1e-9)
+
+
+def test_launch_shape_does_not_change_results():
+    g = O.golden("cube60b")
+    outs = []
+    for block, blocks in ((64, 7), (256, 0), (128, 33)):
+        with U.Simulation(g["cfg"], block=block, blocks=blocks) as sim:
+            sim.run()
+            outs.append((sorted_rows(sim.detp()[0]), sim.energies()))
+    for rows, e in outs[1:]:
+        assert np.array_equal(rows.view(np.uint32), outs[0][0].view(np.uint32))
+        assert e[1] == outs[0][1][1]
+
+
+def test_full_size_heterogeneous_reflective_properties():
+    """The 1e8-photon heterogeneous reflective workload at 1e7 photons: energy balance, and the
+    detections of a 2e5-photon prefix equal the oracle's bitwise."""
+    g, prob = O.golden_problem("c2")
+    cfg = json.loads(json.dumps(g["cfg"]))
+    cfg["Session"]["Photons"] = 10_000_000
+    say("full-size c2 1e7")
+    with U.Simulation(cfg) as sim:
+        sim.run()
+        esc, tot, dep = sim.energies()
+        detected, saved = sim.detected()
+        sim.reset()
+        sim.run(0, 200000)
+        rows = sorted_rows(sim.detp()[0])
+    assert tot == pytest.approx(1e7)
+    assert abs(dep + esc - tot) <= 1e-6 * tot
+    assert 4000e-6 < detected / 1e7 < 5000e-6
+    o = O.run(prob, nphoton=200000, nthreads=8)
+    assert np.array_equal(rows.view(np.uint32), sorted_rows(o["detp"]).view(np.uint32))

@@ -1,0 +1,151 @@
+"""Host side of the drop-in boundary, checked without a GPU.
+
+* libumcx.so loads and exports every function include/umcx.h declares;
+* the label volume (shapes, JData volumes incl. zlib+base64, detector masks) equals the
+  reference's own --dumpmask output for every golden case, and the --dumpmask file is
+  byte-identical;
+* command-line flags, presets and --json merges expand to the reference's --dumpjson config;
+* the libm replicas equal the host C library bit for bit on every argument the photon loop can
+  form (run on the CPU here; the GPU build is checked in test_gpu_parity.py);
+* seed words equal glibc srand()/rand();
+* failures come back as error codes with the reference's messages, never as exceptions;
+* without a GPU the simulation path fails loudly instead of falling back to the CPU.
+"""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import umcx_amd as U
+from umcx_amd import _native, bjdata
+
+ROOT = O.ROOT
+GOLDEN = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel", "c2",
+          "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx"]
+BENCH_ARGV = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel",
+              "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx"]
+
+
+def header_functions():
+    with open(os.path.join(ROOT, "include", "umcx.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    c_part = text.split("#ifdef __cplusplus\n}")[0]
+    return sorted(set(re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(\w+)\s*\(", c_part, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert "MCX_run_cmd" in names and "MCX_run_json" in names and len(names) >= 20
+    L = _native.lib()
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_native.SIGNATURES)
+    nm = subprocess.run(["nm", "-D", "--defined-only", _native.LIB_PATH], capture_output=True, text=True).stdout
+    assert "_Z18MCX_run_simulationPPciiPPv" in nm     # C++ linkage entry of src/umcx.cpp:783
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_domain_matches_reference_dumpmask(name):
+    g = O.golden(name)
+    lab = U.domain(g["cfg"])
+    assert lab.shape == g["mask"].shape
+    assert np.array_equal(lab, g["mask"])
+
+
+@pytest.mark.parametrize("name", BENCH_ARGV)
+def test_cli_expansion_matches_reference_dumpjson(name):
+    g = O.golden(name)
+    cfg = U.config_from_argv(g["argv"])
+    assert cfg == g["cfg"]
+
+
+@pytest.mark.parametrize("name", ["cube60b", "c3", "skinvessel", "zipvol"])
+def test_dumpmask_file_is_byte_compatible(name, tmp_path):
+    g = O.golden(name)
+    argv = list(g["argv"])
+    if "INPUT.json" in argv:
+        (tmp_path / "INPUT.json").write_text(json.dumps(g["cfg"]))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        assert U.run_cmd(argv + ["--dumpmask"]) == 0, U.last_error()
+    finally:
+        os.chdir(cwd)
+    path = tmp_path / f"{name}_vol.bnii"
+    raw = path.read_bytes()
+    assert len(raw) == int(g["mask_size"])
+    assert raw[:128] == bytes(g["mask_head"])
+    vol = bjdata.load_bnii(str(path))[..., 0]
+    assert np.array_equal(vol, g["mask"])
+
+
+def test_libm_replicas_match_host_libm_on_reachable_inputs():
+    inputs = O.reachable_inputs()
+    for fn, x in inputs.items():
+        mine = U.libm_eval(fn, x)
+        ref = O.host_libm(fn, x)
+        assert np.array_equal(mine.view(np.uint32), ref.view(np.uint32)), f"fn {fn}"
+    # sin/cos also take acos outputs (src/umcx.cpp:372): every float in [0, pi]
+    th = np.arange(0, np.float32(np.pi).view(np.uint32) + 1, dtype=np.uint32).view(np.float32)
+    for fn in (1, 2):
+        assert np.array_equal(U.libm_eval(fn, th).view(np.uint32), O.host_libm(fn, th).view(np.uint32))
+
+
+def test_replica_check_has_teeth():
+    """(float)log(double) differs from glibc's logf on ~61k reachable inputs; the replica must not."""
+    x = O.reachable_inputs()[0]
+    cr = np.log(x.astype(np.float64)).astype(np.float32)
+    ref = O.host_libm(0, x)
+    assert np.count_nonzero(cr.view(np.uint32) != ref.view(np.uint32)) > 50000
+
+
+def test_seed_words():
+    with open(os.path.join(ROOT, "tests", "golden", "seed_words.json")) as f:
+        kat = json.load(f)
+    for seed, words in kat.items():
+        assert U.seed_words(int(seed)) == words
+
+
+def test_error_paths_return_codes(tmp_path):
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        assert U.run_cmd(["--bench", "nosuch"]) == 1
+        assert U.last_error() == "benchmark nosuch is not found"
+        assert U.run_cmd(["--bench", "cube60", "-n"]) == 1
+        assert "incomplete input parameter: -n" in U.last_error()
+        assert U.run_cmd(["a.json", "b.json"]) == 1
+        assert U.last_error() == "must use -flag or --flag to use more than 1 input"
+        bad_shape = U.config_from_argv(["--bench", "cube60"])
+        bad_shape["Shapes"].append({"Torus": {"R": 3}})
+        assert U.run_json(bad_shape) == 1
+        assert U.last_error() == "shape constructTorus is not supported"
+        nophotons = U.config_from_argv(["--bench", "cube60"])
+        del nophotons["Session"]["Photons"]
+        assert U.run_json(nophotons) == 1
+        assert "type must be number" in U.last_error()
+        assert U.run_json("{not json") == 1
+        assert "parse_error" in U.last_error()
+    finally:
+        os.chdir(cwd)
+
+
+def test_dumpjson_and_help_succeed(capfd):
+    assert U.run_cmd(["--bench", "cube60", "--json", '{"Optode":{"Source":{"Type":"isotropic"}}}', "--dumpjson"]) == 0
+    out = capfd.readouterr().out
+    assert '"ID": "cube60"' in out and "isotropic" in out
+    assert U.run_cmd(["-h"]) == 0
+    assert "cube60" in capfd.readouterr().out
+
+
+@pytest.mark.skipif(U.device_count() > 0, reason="checks the no-GPU behaviour")
+def test_simulation_without_gpu_fails_loudly():
+    with pytest.raises(U.UmcxError, match="no HIP device"):
+        U.Simulation(U.config_from_argv(["--bench", "cube60", "-n", "100"]))
+    assert U.run_cmd(["--bench", "cube60", "-n", "100"]) == 1
+    assert "no HIP device" in U.last_error()

@@ -1,0 +1,90 @@
+"""ctypes binding of libumcx.so (C ABI declared in include/umcx.h).
+
+The library is built in-tree (``umcx_amd/lib/libumcx.so``, see ``umcx_amd/build.py``). There is
+no fallback: when it is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libumcx.so")
+
+
+class UmcxLaunch(C.Structure):
+    _fields_ = [("device", C.c_int32), ("block", C.c_int32), ("blocks", C.c_int32),
+                ("tally_mode", C.c_int32), ("trace", C.c_int32)]
+
+
+class UmcxInfo(C.Structure):
+    _fields_ = [("dims", C.c_uint32 * 4), ("nphoton", C.c_uint64),
+                ("mediumnum", C.c_int32), ("detnum", C.c_int32), ("detp_columns", C.c_int32),
+                ("ppath_columns", C.c_int32), ("outputtype", C.c_int32), ("isnormalized", C.c_int32),
+                ("issavevol", C.c_int32), ("issavedet", C.c_int32), ("isreflect", C.c_int32),
+                ("srctype", C.c_int32), ("tstart", C.c_float), ("tend", C.c_float), ("rtstep", C.c_float),
+                ("unitinmm", C.c_float), ("seeds", C.c_uint32 * 4), ("block", C.c_int32), ("blocks", C.c_int32),
+                ("tally_mode", C.c_int32), ("label_bytes", C.c_int32)]
+
+
+# every exported symbol and its signature: (restype, argtypes)
+SIGNATURES = {
+    "MCX_run_cmd": (C.c_int, [C.POINTER(C.c_char_p), C.c_int]),
+    "MCX_run_json": (C.c_int, [C.c_char_p]),
+    "umcx_last_error": (C.c_char_p, []),
+    "umcx_version": (C.c_char_p, []),
+    "umcx_device_count": (C.c_int, []),
+    "umcx_session_from_json": (C.c_int, [C.c_char_p, C.POINTER(UmcxLaunch), C.POINTER(C.c_void_p)]),
+    "umcx_session_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(UmcxLaunch), C.POINTER(C.c_void_p)]),
+    "umcx_session_info": (C.c_int, [C.c_void_p, C.POINTER(UmcxInfo)]),
+    "umcx_session_reset": (C.c_int, [C.c_void_p]),
+    "umcx_session_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(C.c_double)]),
+    "umcx_session_fluence": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "umcx_session_fluence_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "umcx_session_detected": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "umcx_session_detp": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "umcx_session_energies": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "umcx_session_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "umcx_session_normalizer": (C.c_float, [C.c_void_p, C.c_double]),
+    "umcx_session_free": (None, [C.c_void_p]),
+    "umcx_domain_from_json": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
+    "umcx_config_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "umcx_seed_words": (C.c_int, [C.c_int32, C.POINTER(C.c_uint32)]),
+    "umcx_libm_host": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64]),
+    "umcx_libm_device": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32]),
+    "umcx_free": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+class UmcxError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libumcx.so once; raises when the native library is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UmcxError(f"native library missing: {LIB_PATH} (build it with `python -m umcx_amd.build`)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise UmcxError(lib().umcx_last_error().decode("utf-8", "replace"))
+
+
+def argv_array(args: list[str]):
+    arr = (C.c_char_p * (len(args) + 1))()
+    for i, a in enumerate(args):
+        arr[i] = a.encode()
+    arr[len(args)] = None
+    return arr

@@ -1,0 +1,288 @@
+// capi.cpp -- the C ABI of libumcx.so (include/umcx.h) and MCX_run_simulation.
+#include "../../include/umcx.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <string>
+
+#include "bjdata.h"
+#include "config.h"
+#include "glibc_libm.h"
+#include "runtime.h"
+
+struct umcx_session {
+    std::unique_ptr<umcx::Session> s;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        g_last_error.clear();
+        return f();
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return 1;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return 1;
+    }
+}
+
+umcx::LaunchConfig to_launch(const umcx_launch* l) {
+    umcx::LaunchConfig lc;
+    if (l) {
+        lc.device = l->device;
+        lc.block = l->block;
+        lc.blocks = l->blocks;
+        lc.tally_mode = l->tally_mode;
+        lc.trace = l->trace != 0;
+    }
+    return lc;
+}
+
+// Normaliser of src/umcx.cpp:820: double arithmetic, stored as float
+float normalizer_of(const umcx::Problem& p, double energytotal) {
+    if (p.outputtype == umcx::kEnergy) return (float)(1.f / energytotal);
+    if (p.outputtype == umcx::kFluenceRate)
+        return (float)(p.rtstep / (energytotal * p.unitinmm * p.unitinmm));
+    return (float)(1.f / (energytotal * p.unitinmm * p.unitinmm));
+}
+
+std::string out_name(const umcx::Problem& p, const char* suffix, const char* dflt) {
+    return p.session_id.empty() ? std::string(dflt) : p.session_id + suffix;
+}
+
+}  // namespace
+
+int MCX_run_simulation(char* argv[], int argn, int nlhs, void* plhs[]) {
+    (void)nlhs;
+    (void)plhs;
+    return guarded([&]() -> int {
+        if (argn < 0) throw std::runtime_error("BJData (MATLAB) input buffers are not supported by this build");
+        umcx::ParsedInput in = umcx::parse_argv(argn, argv);
+        if (in.action == umcx::Action::Help) {
+            std::cout << umcx::help_text() << std::flush;
+            return 0;
+        }
+        if (in.action == umcx::Action::DumpJson) {
+            uint32_t nx, ny, nz;
+            std::vector<int32_t> lab;
+            umcx::build_domain(in.cfg, nx, ny, nz, lab);   // the reference builds the domain first
+            std::cout << in.cfg.dump(2) << std::endl;
+            return 0;
+        }
+        if (in.action == umcx::Action::DumpMask) {
+            uint32_t nx, ny, nz;
+            std::vector<int32_t> lab;
+            umcx::build_domain(in.cfg, nx, ny, nz, lab);
+            const umcx::Json* s = in.cfg.find("Session");
+            std::string name = (s && s->contains("ID")) ? s->at("ID").as_string() + "_vol.bnii" : std::string("vol.bnii");
+            umcx::write_bnii_int32(name, lab.data(), nx, ny, nz, 1);
+            return 0;
+        }
+        umcx::Problem prob = umcx::make_problem(in.cfg);
+        umcx::LaunchConfig lc;
+        lc.device = prob.deviceid > 0 ? prob.deviceid - 1 : 0;
+        if (umcx::device_count() <= 0) throw std::runtime_error("no HIP device available: the umcx GPU path needs an MI355X (gfx950)");
+        if (hipSetDevice(lc.device) != hipSuccess || hipFree(nullptr) != hipSuccess)
+            throw std::runtime_error("cannot initialise HIP device " + std::to_string(lc.device));
+        // timing window of src/umcx.cpp:812-840: device upload, the photon loop, read-back
+        auto t0 = std::chrono::steady_clock::now();
+        umcx::Session sess(prob, lc);
+        sess.run(0, prob.nphoton);
+        std::vector<float> flu(sess.fluence_len());
+        sess.fetch_fluence(flu.data());
+        uint32_t saved = sess.saved();
+        std::vector<float> rows((size_t)saved * std::max(prob.detp_columns(), 1));
+        sess.fetch_detp(rows.data(), nullptr, nullptr);
+        double e[3];
+        sess.energies(e);
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        double escape = e[0], total = e[1];
+        float norm = normalizer_of(prob, total);
+        printf("simulated energy %.2f, speed %.2f photon/ms, duration %.6f ms, normalizer %.6f, detected %d, absorbed %.6f%%\n",
+               total, (double)prob.nphoton / ms, ms, norm, (int)saved, (total - escape) / total * 100.);
+        fflush(stdout);
+        if (prob.issavevol) {
+            if (prob.isnormalized && norm != 1.f)
+                for (float& v : flu) v *= norm;
+            umcx::write_bnii_float(out_name(prob, ".bnii", "output.bnii"), flu.data(), prob.nx, prob.ny, prob.nz,
+                                   (uint32_t)prob.maxgate);
+        }
+        if (prob.issavedet) {
+            umcx::DetpInfo info{prob.mediumnum, prob.detnum, prob.detp_columns(), sess.detected(), saved, prob.unitinmm};
+            umcx::write_detp(out_name(prob, "_detp.jdb", "output_detp.jdb"), info, rows.data());
+        }
+        return 0;
+    });
+}
+
+extern "C" {
+
+int MCX_run_cmd(char* argv[], int argn) { return MCX_run_simulation(argv, argn); }
+
+int MCX_run_json(char* jsoninput) {
+    char* flags[] = {(char*)"", (char*)"--json", jsoninput};
+    return MCX_run_simulation(flags, 3);
+}
+
+const char* umcx_last_error(void) { return g_last_error.c_str(); }
+const char* umcx_version(void) { return "umcx-mi355x 0.1 (gfx950)"; }
+int umcx_device_count(void) { return umcx::device_count(); }
+
+int umcx_session_from_json(const char* json, const umcx_launch* launch, umcx_session** out) {
+    return guarded([&]() -> int {
+        if (!json || !out) throw std::runtime_error("umcx_session_from_json: null argument");
+        umcx::Problem p = umcx::make_problem(umcx::Json::parse(json));
+        auto* s = new umcx_session;
+        try {
+            s->s.reset(new umcx::Session(p, to_launch(launch)));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return 0;
+    });
+}
+
+int umcx_session_from_argv(char** argv, int argn, const umcx_launch* launch, umcx_session** out) {
+    return guarded([&]() -> int {
+        if (!out) throw std::runtime_error("umcx_session_from_argv: null argument");
+        umcx::ParsedInput in = umcx::parse_argv(argn, argv);
+        umcx::Problem p = umcx::make_problem(in.cfg);
+        auto* s = new umcx_session;
+        try {
+            s->s.reset(new umcx::Session(p, to_launch(launch)));
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = s;
+        return 0;
+    });
+}
+
+int umcx_session_info(const umcx_session* s, umcx_info* info) {
+    return guarded([&]() -> int {
+        if (!s || !info) throw std::runtime_error("umcx_session_info: null argument");
+        const umcx::Problem& p = s->s->problem();
+        memset(info, 0, sizeof *info);
+        info->dims[0] = p.nx; info->dims[1] = p.ny; info->dims[2] = p.nz; info->dims[3] = (uint32_t)p.maxgate;
+        info->nphoton = p.nphoton;
+        info->mediumnum = p.mediumnum; info->detnum = p.detnum;
+        info->detp_columns = p.detp_columns(); info->ppath_columns = p.ppath_columns();
+        info->outputtype = p.outputtype; info->isnormalized = p.isnormalized; info->issavevol = p.issavevol;
+        info->issavedet = p.issavedet; info->isreflect = p.isreflect; info->srctype = p.srctype;
+        info->tstart = p.tstart; info->tend = p.tend; info->rtstep = p.rtstep; info->unitinmm = p.unitinmm;
+        for (int i = 0; i < 4; i++) info->seeds[i] = p.seeds[i];
+        info->block = s->s->block(); info->blocks = s->s->blocks(); info->tally_mode = s->s->tally_mode();
+        info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
+        return 0;
+    });
+}
+
+int umcx_session_reset(umcx_session* s) {
+    return guarded([&]() -> int { s->s->reset(); return 0; });
+}
+
+int umcx_session_run(umcx_session* s, uint64_t first, uint64_t count, double* kernel_ms) {
+    return guarded([&]() -> int {
+        umcx::RunStats st = s->s->run(first, count);
+        if (kernel_ms) *kernel_ms = st.kernel_ms;
+        return 0;
+    });
+}
+
+int umcx_session_fluence(const umcx_session* s, float* out, uint64_t n) {
+    return guarded([&]() -> int {
+        if (n < s->s->fluence_len()) throw std::runtime_error("umcx_session_fluence: buffer too small");
+        s->s->fetch_fluence(out);
+        return 0;
+    });
+}
+
+int umcx_session_fluence_device(const umcx_session* s, float** dptr) {
+    return guarded([&]() -> int { *dptr = s->s->device_fluence(); return 0; });
+}
+
+int umcx_session_detected(const umcx_session* s, uint32_t* detected, uint32_t* saved) {
+    return guarded([&]() -> int {
+        uint32_t d = s->s->detected();
+        if (detected) *detected = d;
+        if (saved) *saved = s->s->saved();
+        return 0;
+    });
+}
+
+int umcx_session_detp(const umcx_session* s, float* rows, uint32_t* ids, float* weights) {
+    return guarded([&]() -> int { s->s->fetch_detp(rows, ids, weights); return 0; });
+}
+
+int umcx_session_energies(const umcx_session* s, double out[3]) {
+    return guarded([&]() -> int { s->s->energies(out); return 0; });
+}
+
+int umcx_session_trace(const umcx_session* s, void* out, uint64_t n) {
+    return guarded([&]() -> int { s->s->fetch_trace((umcx::TraceRec*)out, n); return 0; });
+}
+
+float umcx_session_normalizer(const umcx_session* s, double energytotal) {
+    return normalizer_of(s->s->problem(), energytotal);
+}
+
+void umcx_session_free(umcx_session* s) { delete s; }
+
+int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels) {
+    return guarded([&]() -> int {
+        std::vector<int32_t> lab;
+        umcx::build_domain(umcx::Json::parse(json), dims[0], dims[1], dims[2], lab);
+        *labels = (int32_t*)malloc(lab.size() * 4);
+        if (!*labels) throw std::runtime_error("out of host memory");
+        memcpy(*labels, lab.data(), lab.size() * 4);
+        return 0;
+    });
+}
+
+int umcx_config_from_argv(char** argv, int argn, char** json_out) {
+    return guarded([&]() -> int {
+        umcx::ParsedInput in = umcx::parse_argv(argn, argv);
+        std::string s = in.cfg.dump(2);
+        *json_out = (char*)malloc(s.size() + 1);
+        memcpy(*json_out, s.c_str(), s.size() + 1);
+        return 0;
+    });
+}
+
+int umcx_seed_words(int32_t seed, uint32_t out[4]) {
+    umcx::glibc_seed_words(seed, out);
+    return 0;
+}
+
+int umcx_libm_host(int32_t fn, const float* in, float* out, uint64_t n) {
+    namespace L = umcx::libm;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)n; i++) {
+        float x = in[i];
+        out[i] = fn == 0 ? L::logf_host(x) : fn == 1 ? L::sinf_host(x) : fn == 2 ? L::cosf_host(x) : L::acosf_host(x);
+    }
+    return 0;
+}
+
+int umcx_libm_device(int32_t fn, const float* in, float* out, uint64_t n, int32_t device) {
+    return guarded([&]() -> int { umcx::device_libm_eval(fn, in, out, n, device); return 0; });
+}
+
+void umcx_free(void* p) { free(p); }
+
+}  // extern "C"

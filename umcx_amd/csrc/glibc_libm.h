@@ -1,0 +1,204 @@
+// glibc_libm.h -- bit-exact device replicas of the host C library's single-precision
+// logf, sinf, cosf and acosf, plus nextafterf.
+//
+// Why: umcx's photon trajectories go through libm (src/umcx.cpp:213 logf, :437-438 sinf/cosf,
+// :232,:372,:374 acosf, :31 nextafterf). glibc 2.35 is not correctly rounded on 0.7-7.7% of the
+// arguments the photon loop can reach, so a correctly rounded GPU libm would send almost every
+// photon down a different path. These functions reproduce the host library instead, from the
+// published designs it implements:
+//   * logf, sinf, cosf: the ARM optimized-routines algorithms (Szabolcs Nagy, 2017-2018) that
+//     glibc uses since 2.28 -- a 16-entry (invc, log c) table with a cubic in double for logf;
+//     double range reduction by pi/2 and even/odd polynomials for sinf/cosf. The x86-64 glibc
+//     selects its FMA-compiled variants on FMA-capable hosts; the fused forms are used here.
+//   * acosf: the fdlibm single-precision algorithm (Sun Microsystems, 1993) that glibc's
+//     sysdeps/ieee754/flt-32 keeps: rational approximation of asin on [0, 0.5] and the
+//     half-angle reduction above it, in float.
+// Verified against the host libm on every float of logf (0,1], sinf/cosf [0,8) and acosf
+// [-1,1] (0 mismatches, FMA and non-FMA orders alike), and on the GPU over every argument the
+// photon loop can form (tests/test_libm_replica.py).
+//
+// Only the argument ranges the photon loop uses are implemented: logf for positive normal
+// inputs (u + FLT_EPSILON >= 2^-23), sinf/cosf for |x| < 120, acosf on [-1, 1].
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define UMCX_HD __host__ __device__ inline
+#else
+#define UMCX_HD inline
+#include <math.h>
+#include <string.h>
+#endif
+
+namespace umcx {
+namespace libm {
+
+UMCX_HD uint32_t f2u(float f) {
+#if defined(__HIPCC__)
+    return __builtin_bit_cast(uint32_t, f);
+#else
+    uint32_t u; memcpy(&u, &f, 4); return u;
+#endif
+}
+UMCX_HD float u2f(uint32_t u) {
+#if defined(__HIPCC__)
+    return __builtin_bit_cast(float, u);
+#else
+    float f; memcpy(&f, &u, 4); return f;
+#endif
+}
+UMCX_HD double dfma(double a, double b, double c) {
+#if defined(__HIPCC__)
+    return __builtin_fma(a, b, c);
+#else
+    return fma(a, b, c);
+#endif
+}
+
+// ---- logf: log(x) = log1p(z/c - 1) + log(c) + k ln2, with x = 2^k z, z near c ----
+struct LogfEntry { double invc, logc; };
+#if defined(__HIPCC__)
+__device__ __constant__
+#endif
+static const LogfEntry kLogfTab[16] = {
+    {0x1.661ec79f8f3bep+0, -0x1.57bf7808caadep-2}, {0x1.571ed4aaf883dp+0, -0x1.2bef0a7c06ddbp-2},
+    {0x1.49539f0f010bp+0, -0x1.01eae7f513a67p-2},  {0x1.3c995b0b80385p+0, -0x1.b31d8a68224e9p-3},
+    {0x1.30d190c8864a5p+0, -0x1.6574f0ac07758p-3}, {0x1.25e227b0b8eap+0, -0x1.1aa2bc79c81p-3},
+    {0x1.1bb4a4a1a343fp+0, -0x1.a4e76ce8c0e5ep-4}, {0x1.12358f08ae5bap+0, -0x1.1973c5a611ccp-4},
+    {0x1.0953f419900a7p+0, -0x1.252f438e10c1ep-5}, {0x1p+0, 0x0p+0},
+    {0x1.e608cfd9a47acp-1, 0x1.aa5aa5df25984p-5},  {0x1.ca4b31f026aap-1, 0x1.c5e53aa362eb4p-4},
+    {0x1.b2036576afce6p-1, 0x1.526e57720db08p-3}, {0x1.9c2d163a1aa2dp-1, 0x1.bc2860d22477p-3},
+    {0x1.886e6037841edp-1, 0x1.1058bc8a07ee1p-2}, {0x1.767dcf5534862p-1, 0x1.4043057b6ee09p-2},
+};
+
+UMCX_HD float logf_host(float x) {
+    const double ln2 = 0x1.62e42fefa39efp-1;
+    const double a0 = -0x1.00ea348b88334p-2, a1 = 0x1.5575b0be00b6ap-2, a2 = -0x1.ffffef20a4123p-2;
+    uint32_t ix = f2u(x);
+    if (ix == 0x3f800000u) return 0.f;
+    uint32_t t = ix - 0x3f330000u;            // subintervals start at 0x3f330000 (~0.7)
+    int idx = (int)((t >> 19) & 15u);
+    int k = (int32_t)t >> 23;
+    float zf = u2f(ix - (t & 0xff800000u));
+    const LogfEntry e = kLogfTab[idx];
+    double r = dfma((double)zf, e.invc, -1.0);
+    double y0 = dfma((double)k, ln2, e.logc);
+    double r2 = r * r;
+    double y = dfma(a1, r, a2);
+    y = dfma(a0, r2, y);
+    y = dfma(y, r2, y0 + r);
+    return (float)y;
+}
+
+// ---- sinf/cosf: x = n pi/2 + r, |r| <= pi/4, then odd/even polynomials in double ----
+struct SinCosPoly { double c0, c1, c2, c3, c4; };
+UMCX_HD uint32_t abstop12(float x) { return (f2u(x) >> 20) & 0x7ffu; }
+
+UMCX_HD float sincos_poly(double x, double x2, bool cos_branch, bool neg_even) {
+    // s1..s3 odd coefficients; c0..c4 even coefficients (negated for quadrants 2,3)
+    const double s1 = -0x1.555545995a603p-3, s2 = 0x1.1107605230bc4p-7, s3 = -0x1.994eb3774cf24p-13;
+    if (!cos_branch) {
+        double x3 = x * x2;
+        double p = dfma(x2, s3, s2);
+        double x7 = x3 * x2;
+        double s = dfma(x3, s1, x);
+        return (float)dfma(x7, p, s);
+    }
+    const double sg = neg_even ? -1.0 : 1.0;
+    const double c0 = sg * 0x1p0, c1 = sg * -0x1.ffffffd0c621cp-2, c2 = sg * 0x1.55553e1068f19p-5,
+                 c3 = sg * -0x1.6c087e89a359dp-10, c4 = sg * 0x1.99343027bf8c3p-16;
+    double x4 = x2 * x2;
+    double q2 = dfma(x2, c4, c3);
+    double q1 = dfma(x2, c1, c0);
+    double x6 = x4 * x2;
+    double c = dfma(x4, c2, q1);
+    return (float)dfma(x6, q2, c);
+}
+
+// Quadrant reduction shared by sinf and cosf of one argument; |y| < 120.
+UMCX_HD double reduce_quadrant(double x, int* n) {
+    const double hpi_inv_scaled = 0x1.45F306DC9C883p+23;   // 2/pi * 2^24
+    const double hpi = 0x1.921FB54442D18p0;
+    double r = x * hpi_inv_scaled;
+    int q = ((int32_t)r + 0x800000) >> 24;
+    *n = q;
+    return dfma(-(double)q, hpi, x);
+}
+
+// sinf(y) and cosf(y) of the same argument, each bitwise equal to the separate host calls.
+UMCX_HD void sincosf_host(float y, float* s, float* c) {
+    double x = y;
+    if (abstop12(y) < abstop12(0x1.921FB6p-1f)) {          // |y| < pi/4
+        if (abstop12(y) < abstop12(0x1p-12f)) { *s = y; *c = 1.0f; return; }
+        double x2 = x * x;
+        *s = sincos_poly(x, x2, false, false);
+        *c = sincos_poly(x, x2, true, false);
+        return;
+    }
+    int n;
+    double r = reduce_quadrant(x, &n);
+    const double sgn = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;   // {1,-1,-1,1}
+    double rs = r * sgn, r2 = r * r;
+    bool neg = (n & 2) != 0;
+    *s = sincos_poly(rs, r2, (n & 1) != 0, neg);
+    *c = sincos_poly(rs, r2, ((n ^ 1) & 1) != 0, neg);
+}
+UMCX_HD float sinf_host(float y) { float s, c; sincosf_host(y, &s, &c); return s; }
+UMCX_HD float cosf_host(float y) { float s, c; sincosf_host(y, &s, &c); return c; }
+
+// ---- acosf: fdlibm single precision ----
+UMCX_HD float acos_rational(float z) {
+    const float pS0 = 1.6666667163e-01f, pS1 = -3.2556581497e-01f, pS2 = 2.0121252537e-01f,
+                pS3 = -4.0055535734e-02f, pS4 = 7.9153501429e-04f, pS5 = 3.4793309169e-05f,
+                qS1 = -2.4033949375e+00f, qS2 = 2.0209457874e+00f, qS3 = -6.8828397989e-01f,
+                qS4 = 7.7038154006e-02f;
+    float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
+    float q = 1.0f + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
+    return p / q;
+}
+UMCX_HD float sqrtf_cr(float x) {
+#if defined(__HIPCC__)
+    return __builtin_sqrtf(x);      // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
+#else
+    return sqrtf(x);
+#endif
+}
+UMCX_HD float acosf_host(float x) {
+    const float pi = 3.1415925026e+00f, pio2_hi = 1.5707962513e+00f, pio2_lo = 7.5497894159e-08f;
+    int32_t hx = (int32_t)f2u(x);
+    int32_t ix = hx & 0x7fffffff;
+    if (ix == 0x3f800000) return hx > 0 ? 0.0f : pi + 2.0f * pio2_lo;
+    if (ix > 0x3f800000) return (x - x) / (x - x);
+    if (ix < 0x3f000000) {                                   // |x| < 0.5
+        if (ix <= 0x23000000) return pio2_hi + pio2_lo;
+        float r = acos_rational(x * x);
+        return pio2_hi - (x - (pio2_lo - x * r));
+    }
+    if (hx < 0) {                                            // x <= -0.5
+        float z = (1.0f + x) * 0.5f;
+        float r = acos_rational(z);
+        float s = sqrtf_cr(z);
+        float w = r * s - pio2_lo;
+        return pi - 2.0f * (s + w);
+    }
+    float z = (1.0f - x) * 0.5f;                             // x >= 0.5
+    float s = sqrtf_cr(z);
+    float df = u2f(f2u(s) & 0xfffff000u);
+    float c = (z - df * df) / (s + df);
+    float r = acos_rational(z);
+    float w = r * s + c;
+    return 2.0f * (df + w);
+}
+
+// ---- nextafterf(x, y) by bit stepping (IEEE 754 semantics, denormals kept) ----
+UMCX_HD float nextafterf_host(float x, float y) {
+    if (x != x || y != y) return x + y;
+    if (x == y) return y;
+    uint32_t ux = f2u(x);
+    if ((ux & 0x7fffffffu) == 0) return u2f((f2u(y) & 0x80000000u) | 1u);
+    bool up = (x < y) == (x > 0.f);      // move away from zero when heading outward
+    return u2f(up ? ux + 1u : ux - 1u);
+}
+
+}  // namespace libm
+}  // namespace umcx

@@ -1,0 +1,726 @@
+// photon_kernel.hip -- gfx950 photon-transport kernel (plain form) and its Session runtime.
+//
+// One lane owns one photon at a time. Waves are persistent: a lane whose photon ends takes the
+// next photon index from a device cursor (one atomic per wave per refill), so lanes stay busy
+// while their neighbours' photons live on. Photon i is a pure function of (seed words, i,
+// problem) -- its xorshift128+ stream is reseeded from i (src/umcx.cpp:761) -- so the order in
+// which lanes take photons does not change any photon's trajectory.
+//
+// Arithmetic follows the reference's FP32 operation order exactly (no contraction: built with
+// -ffp-contract=off; division and sqrt correctly rounded; denormals kept), and the
+// trajectory-affecting libm calls use the bit-exact replicas of glibc_libm.h. The photon weight
+// only decays through expf (src/umcx.cpp:326) and never steers a photon, so expf is the
+// device's own.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "glibc_libm.h"
+#include "runtime.h"
+
+#define HIP_OK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                   \
+    } while (0)
+
+namespace umcx {
+namespace dev {
+
+constexpr float kOneOverC0 = 3.335640951981520e-12f;   // src/umcx.cpp:29
+constexpr float kPi = 3.1415926535897932385f;           // :30
+constexpr float kEps = 1.19209290e-07f;                 // FLT_EPSILON
+constexpr uint32_t kDet = 0x80000000u, kMed = 0x7FFFFFFFu;
+
+struct KernelArgs {
+    uint32_t nx, ny, nz, nyz;
+    uint64_t nxyz;
+    int32_t maxgate, outputtype, savedetflag, mediumnum, detnum, srctype, issavevol, ppathlen, detlen;
+    float tstart, tend, rtstep, pad0;
+    float4 srcpos, srcdir, srcp1, srcp2;
+    uint32_t seeds[4];
+    uint64_t first, count;
+    const void* labels;
+    const float4* media;
+    const float4* detpos;
+    float* fluence;
+    float* detp;
+    uint32_t* detid;
+    float* detw;
+    uint32_t* ndet;
+    uint32_t maxdet;
+    double* energy;
+    unsigned long long* cursor;
+    TraceRec* trace;
+};
+
+// ---------------- xorshift128+ (src/umcx.cpp:186-215) ----------------
+struct Rng {
+    uint64_t t0, t1;
+};
+__device__ __forceinline__ void rng_seed(Rng& r, uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    r.t0 = ((uint64_t)a << 32) | b;
+    r.t1 = ((uint64_t)c << 32) | d;
+}
+__device__ __forceinline__ float rng_u01(Rng& r) {
+    uint64_t s0 = r.t1, s1 = r.t0;
+    r.t0 = s0;
+    s1 ^= s1 << 23;
+    r.t1 = s1 ^ s0 ^ (s1 >> 18) ^ (s0 >> 5);
+    uint32_t lo = (uint32_t)(r.t1 + s0);
+    return __builtin_bit_cast(float, 0x3F800000u | (lo >> 9)) - 1.0f;
+}
+__device__ __forceinline__ float rng_scatlen(Rng& r) {
+    return -libm::logf_host(rng_u01(r) + kEps);
+}
+
+// ---------------- photon state (src/umcx.cpp:216-220) ----------------
+struct Photon {
+    float px, py, pz, pw;      // position, weight
+    float vx, vy, vz, vw;      // direction, scatter count
+    float rx, ry, rz;          // reciprocal direction
+    float lx, ly, lz, lw;      // scattering length left, time, path since save, weight at save
+    int ix, iy, iz, iw;        // voxel (short semantics) and crossing axis
+    int lastvox;
+    uint32_t med;
+};
+
+template <typename LabelT>
+__device__ __forceinline__ uint32_t load_label(const void* labels, int idx);
+template <>
+__device__ __forceinline__ uint32_t load_label<uint8_t>(const void* labels, int idx) {
+    uint32_t v = ((const uint8_t*)labels)[idx];
+    return (v & 0x7Fu) | ((v & 0x80u) << 24);
+}
+template <>
+__device__ __forceinline__ uint32_t load_label<uint32_t>(const void* labels, int idx) {
+    return ((const uint32_t*)labels)[idx];
+}
+
+__device__ __forceinline__ int to_short(float x) { return (int)(short)(int)x; }
+
+// MCX_volume::index (:128-130)
+__device__ __forceinline__ int vox_index(const KernelArgs& a, int ix, int iy, int iz) {
+    if ((unsigned)ix >= a.nx || (unsigned)iy >= a.ny || (unsigned)iz >= a.nz) return -1;
+    return ix * (int)a.nyz + iy * (int)a.nz + iz;
+}
+
+// rotatevector (:418-428)
+__device__ __forceinline__ void rotate(Photon& p, float st, float ct, float sp, float cp) {
+    float nx, ny, nz;
+    if (p.vz > -1.f + kEps && p.vz < 1.f - kEps) {
+        float t0 = 1.f - p.vz * p.vz;
+        float t1 = st / libm::sqrtf_cr(t0);
+        nx = t1 * (p.vx * p.vz * cp - p.vy * sp) + p.vx * ct;
+        ny = t1 * (p.vy * p.vz * cp + p.vx * sp) + p.vy * ct;
+        nz = -t1 * t0 * cp + p.vz * ct;
+    } else {
+        nx = st * cp;
+        ny = st * sp;
+        nz = (p.vz > 0.f) ? ct : -ct;
+    }
+    float s = 1.f / libm::sqrtf_cr(nx * nx + ny * ny + nz * nz);
+    p.vx = nx * s;
+    p.vy = ny * s;
+    p.vz = nz * s;
+}
+
+// launch (:225-259); returns the launch weight
+__device__ __forceinline__ float launch(Photon& p, Rng& r, const KernelArgs& a) {
+    p.px = a.srcpos.x; p.py = a.srcpos.y; p.pz = a.srcpos.z; p.pw = 1.f;
+    p.vx = a.srcdir.x; p.vy = a.srcdir.y; p.vz = a.srcdir.z; p.vw = 0.f;
+    const int st = a.srctype;
+    if (st == kIsotropic || st == kCone) {
+        float sphi, cphi, sth, cth;
+        libm::sincosf_host((kPi * 2.f) * rng_u01(r), &sphi, &cphi);
+        float th;
+        if (st == kIsotropic) {
+            th = libm::acosf_host(2.f * rng_u01(r) - 1.f);
+        } else {
+            float ca = libm::cosf_host(a.srcp1.x);
+            th = libm::acosf_host(rng_u01(r) * (1.0f - ca) + ca);
+        }
+        libm::sincosf_host(th, &sth, &cth);
+        rotate(p, sth, cth, sphi, cphi);
+    } else if (st == kDisk) {
+        float sphi, cphi;
+        libm::sincosf_host((kPi * 2.f) * rng_u01(r), &sphi, &cphi);
+        float rad = libm::sqrtf_cr(rng_u01(r) * fabsf(a.srcp1.x * a.srcp1.x - a.srcp1.y * a.srcp1.y) + a.srcp1.y * a.srcp1.y);
+        if (p.vz > -1.f + kEps && p.vz < 1.f - kEps) {
+            float q = 1.f - p.vz * p.vz;
+            float b = rad / libm::sqrtf_cr(q);
+            float nx = p.px + b * (p.vx * p.vz * cphi - p.vy * sphi);
+            float ny = p.py + b * (p.vy * p.vz * cphi + p.vx * sphi);
+            float nz = p.pz - b * q * cphi;
+            p.px = nx; p.py = ny; p.pz = nz;
+        } else {
+            p.px += rad * cphi;
+            p.py += rad * sphi;
+        }
+    } else if (st == kPlanar || st == kFourier) {
+        float u = rng_u01(r), v = rng_u01(r);
+        float nx = p.px + u * a.srcp1.x + v * a.srcp2.x;
+        float ny = p.py + u * a.srcp1.y + v * a.srcp2.y;
+        float nz = p.pz + u * a.srcp1.z + v * a.srcp2.z;
+        p.px = nx; p.py = ny; p.pz = nz;
+        if (st == kFourier) {
+            float f1 = floorf(a.srcp1.w), f2 = floorf(a.srcp2.w);
+            float arg = (f1 * u + f2 * v + a.srcp1.w - f1) * kPi * 2.f;
+            p.pw *= (libm::cosf_host(arg) * (1.f - a.srcp2.w + f2) + 1.f) * 0.5f;
+        }
+    }
+    p.rx = 1.f / a.srcdir.x;   // from the nominal direction (:255)
+    p.ry = 1.f / a.srcdir.y;
+    p.rz = 1.f / a.srcdir.z;
+    p.lx = __builtin_nanf("");
+    p.ly = 0.f; p.lz = 0.f; p.lw = p.pw;
+    p.ix = to_short(p.px); p.iy = to_short(p.py); p.iz = to_short(p.pz); p.iw = -1;
+    return p.pw;
+}
+
+// skip (:339-358): bring a photon launched outside the grid to its entry point
+__device__ __forceinline__ bool skip_to_box(Photon& p, const KernelArgs& a) {
+    float tx0 = -p.px * p.rx, ty0 = -p.py * p.ry, tz0 = -p.pz * p.rz;
+    float tx1 = ((float)a.nx - p.px) * p.rx;
+    float ty1 = ((float)a.ny - p.py) * p.ry;
+    float tz1 = ((float)a.nz - p.pz) * p.rz;
+    float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    if (tmax < 0.f || tmin > tmax) {
+        p.lx = tx0; p.ly = ty0; p.lz = tz0; p.lw = tx1;
+        p.pw = tz1;
+        return false;
+    }
+    float nx = p.px + tmin * p.vx, ny = p.py + tmin * p.vy, nz = p.pz + tmin * p.vz;
+    p.px = nx; p.py = ny; p.pz = nz; p.pw = 1.f;
+    p.lx = __builtin_nanf(""); p.ly = 0.f; p.lz = 0.f; p.lw = 1.f;
+    p.ix = to_short(p.px); p.iy = to_short(p.py); p.iz = to_short(p.pz); p.iw = -1;
+    p.lastvox = vox_index(a, p.ix, p.iy, p.iz);
+    return true;
+}
+
+// step (:315-338); returns the voxel the photon is in afterwards (-1 outside)
+__device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a) {
+    const float mua = prop.x, mus = prop.y, n = prop.w;
+    float hx = fabsf(((float)(p.ix + (p.vx > 0.f)) - p.px) * p.rx);
+    float hy = fabsf(((float)(p.iy + (p.vy > 0.f)) - p.py) * p.ry);
+    float hz = fabsf(((float)(p.iz + (p.vz > 0.f)) - p.pz) * p.rz);
+    float dmin = fminf(fminf(hx, hy), hz);
+    p.iw = (dmin == hx) ? 0 : ((dmin == hy) ? 1 : 2);
+    float used = fminf(dmin * mus, p.lx);
+    bool cont = used != p.lx;
+    float d = (mus == 0.f) ? dmin : (used / mus);
+    p.px = p.px + d * p.vx;
+    p.py = p.py + d * p.vy;
+    p.pz = p.pz + d * p.vz;
+    p.pw = p.pw * expf(-mua * d);
+    p.lx -= used;
+    p.ly += d * n * kOneOverC0;
+    p.lz += d;
+    if (cont) {
+        // value selects with unconditional writes keep the photon in registers
+        const int ax = p.iw;
+        p.ix = (int)(short)(p.ix + (ax == 0 ? (p.vx > 0.f ? 1 : -1) : 0));
+        p.iy = (int)(short)(p.iy + (ax == 1 ? (p.vy > 0.f ? 1 : -1) : 0));
+        p.iz = (int)(short)(p.iz + (ax == 2 ? (p.vz > 0.f ? 1 : -1) : 0));
+        return vox_index(a, p.ix, p.iy, p.iz);
+    }
+    return p.lastvox;
+}
+
+// scatter (:364-380)
+__device__ __forceinline__ void scatter(Photon& p, Rng& r, float4 prop) {
+    const float g = prop.z;
+    p.lx = rng_scatlen(r);
+    float sphi, cphi, sth, cth;
+    libm::sincosf_host((2.f * kPi) * rng_u01(r), &sphi, &cphi);
+    float th;
+    if (fabsf(g) > kEps) {
+        float t = (1.f - g * g) / (1.f - g + 2.f * g * rng_u01(r));
+        t = (1.f + g * g - t * t) / (2.f * g);
+        th = libm::acosf_host(fmaxf(-1.f, fminf(1.f, t)));
+    } else {
+        th = libm::acosf_host(2.f * rng_u01(r) - 1.f);
+    }
+    libm::sincosf_host(th, &sth, &cth);
+    rotate(p, sth, cth, sphi, cphi);
+    p.rx = 1.f / p.vx;
+    p.ry = 1.f / p.vy;
+    p.rz = 1.f / p.vz;
+    p.vw += 1.f;
+}
+
+// reflectcoeff (:381-396): unpolarised Fresnel reflectance, 1 under total internal reflection
+__device__ __forceinline__ float fresnel(float vaxis, float n1, float n2) {
+    float ci = fabsf(vaxis);
+    float a2 = n1 * n1, b2 = n2 * n2;
+    float ct2 = 1.f - a2 / b2 * (1.f - ci * ci);
+    if (ct2 > 0.f) {
+        float re = a2 * ci * ci + b2 * ct2;
+        ct2 = libm::sqrtf_cr(ct2);
+        float im = 2.f * n1 * n2 * ci * ct2;
+        float rp = (re - im) / (re + im);
+        re = b2 * ci * ci + a2 * ct2 * ct2;
+        return (rp + (re - im) / (re + im)) * 0.5f;
+    }
+    return 1.f;
+}
+
+// reflect (:405-417) with transmit (:397-404); true when the photon is transmitted
+// Register value barrier: a select between photon fields would otherwise be rewritten as a
+// select of field addresses, which forces the whole photon state out to scratch memory.
+__device__ __forceinline__ float in_reg(float x) {
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
+__device__ __forceinline__ float pick_axis(bool ax, bool ay, float x, float y, float z) {
+    return ax ? in_reg(x) : (ay ? in_reg(y) : in_reg(z));
+}
+
+__device__ __forceinline__ bool boundary(Photon& p, float n1, float n2, Rng& r, int& nv, uint32_t& nm) {
+    const bool ax = p.iw == 0, ay = p.iw == 1, az = !ax && !ay;
+    const float vaxis = pick_axis(ax, ay, p.vx, p.vy, p.vz);
+    float R = fresnel(vaxis, n1, n2);
+    if (R < 1.f && rng_u01(r) > R) {
+        float s = n1 / n2;
+        p.vx *= s; p.vy *= s; p.vz *= s;
+        const float c = pick_axis(ax, ay, p.vx, p.vy, p.vz);
+        const float t = p.vx * p.vx + p.vy * p.vy + p.vz * p.vz - c * c;
+        const float cn = (t < 1.f) ? libm::sqrtf_cr(1.f - t) * (float)((c > 0.f) - (c < 0.f)) : 0.f;
+        p.vx = ax ? cn : p.vx;
+        p.vy = ay ? cn : p.vy;
+        p.vz = az ? cn : p.vz;
+        s = 1.f / libm::sqrtf_cr(p.vx * p.vx + p.vy * p.vy + p.vz * p.vz);
+        p.vx *= s; p.vy *= s; p.vz *= s;
+        return true;
+    }
+    // mirror on the crossed axis (REFLECT_PHOTON, :31)
+    p.vx = ax ? -p.vx : p.vx;  p.rx = ax ? -p.rx : p.rx;
+    p.vy = ay ? -p.vy : p.vy;  p.ry = ay ? -p.ry : p.ry;
+    p.vz = az ? -p.vz : p.vz;  p.rz = az ? -p.rz : p.rz;
+    const float v = -vaxis;
+    const float pos = pick_axis(ax, ay, p.px, p.py, p.pz);
+    const float snapped = libm::nextafterf_host((float)(int)(pos + 0.5f), (float)((v > 0.f) - (v < 0.f)));
+    const int cell = to_short(snapped);
+    p.px = ax ? snapped : p.px;  p.ix = ax ? cell : p.ix;
+    p.py = ay ? snapped : p.py;  p.iy = ay ? cell : p.iy;
+    p.pz = az ? snapped : p.pz;  p.iz = az ? cell : p.iz;
+    nv = p.lastvox;
+    nm = p.med;
+    return false;
+}
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+    return (1ull << __lane_id()) - 1ull;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <bool REFLECT, bool SAVEDET, typename LabelT>
+__global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
+    extern __shared__ float4 lds4[];
+    float4* smedia = lds4;
+    float* sppath = (float*)(lds4 + a.mediumnum);
+    const int tid = threadIdx.x, bd = blockDim.x;
+    for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
+    if (SAVEDET)
+        for (int i = tid; i < a.ppathlen * bd; i += bd) sppath[i] = 0.f;
+    __syncthreads();
+
+    Photon p;
+    Rng r;
+    bool alive = false;
+    uint64_t pid = 0;
+    float w0 = 0.f;
+    double e_esc = 0.0, e_tot = 0.0, e_dep = 0.0, ph_dep = 0.0;
+    const bool savevol = a.issavevol != 0;
+    const bool energy_out = a.outputtype == kEnergy;
+
+    while (true) {
+        // ---- take a new photon when this lane's has ended (one atomic per wave) ----
+        if (!alive) {
+            const unsigned long long need = __ballot(1);
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned long long base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(a.cursor, (unsigned long long)__popcll(need));
+            base = __shfl(base, leader);
+            const unsigned long long k = base + (unsigned long long)__popcll(need & lanemask_lt());
+            if (k >= a.count) break;
+            pid = a.first + k;
+            const uint32_t i32 = (uint32_t)pid;
+            rng_seed(r, a.seeds[0] ^ i32, a.seeds[1] | i32, a.seeds[2] ^ i32, a.seeds[3] | i32);   // :761
+            w0 = launch(p, r, a);
+            ph_dep = 0.0;
+            if (SAVEDET)
+                for (int m = 0; m < a.ppathlen; m++) sppath[m * bd + tid] = 0.f;
+            p.lastvox = vox_index(a, p.ix, p.iy, p.iz);                                              // :262
+            bool entered = p.lastvox >= 0 || skip_to_box(p, a);                                      // :264
+            if (entered && p.lastvox < 0) entered = false;   // entry exactly on a far face
+            if (!entered) {
+                p.med = 0;
+                alive = false;
+                if (a.trace) {
+                    TraceRec& t = a.trace[k];
+                    t.pos[0] = p.px; t.pos[1] = p.py; t.pos[2] = p.pz; t.pos[3] = p.pw;
+                    t.vec[0] = p.vx; t.vec[1] = p.vy; t.vec[2] = p.vz; t.vec[3] = p.vw;
+                    t.len[0] = p.lx; t.len[1] = p.ly; t.len[2] = p.lz; t.len[3] = p.lw;
+                    t.mediaid = 0; t.lastvoxelidx = p.lastvox; t.w0 = w0;
+                }
+                continue;
+            }
+            p.med = load_label<LabelT>(a.labels, p.lastvox);                                         // :269
+            p.lx = rng_scatlen(r);                                                                   // :270
+            alive = true;
+        }
+
+        bool ended = false;
+        // ---- one step of sprint (:282-311) ----
+        if (p.lx > 0.f) {
+            const float4 prop = smedia[p.med & kMed];
+            int nv = step(p, prop, a);
+            if (nv != p.lastvox) {
+                const uint32_t m = p.med & kMed;
+                if (SAVEDET && a.ppathlen && m > 0) sppath[(m - 1) * bd + tid] += p.lz;
+                if (savevol) {
+                    int tshift = (int)fminf((float)(a.maxgate - 1), (float)(int)floorf((p.ly - a.tstart) * a.rtstep));
+                    float dw = p.lw - p.pw;
+                    float val = energy_out ? dw : ((prop.x < kEps) ? (p.lw * p.lz) : dw / prop.x);
+                    ph_dep += (double)dw;
+                    if (tshift >= 0) {
+                        unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
+                    }
+                    p.lw = p.pw;
+                    p.lz = 0.f;
+                }
+                if (p.ly > a.tend) {
+                    ended = true;
+                } else {
+                    uint32_t nm = (nv >= 0) ? load_label<LabelT>(a.labels, nv) : 0u;
+                    const float4 nprop = smedia[nm & kMed];
+                    if (REFLECT && prop.w != nprop.w) {
+                        if (boundary(p, prop.w, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;
+                    } else if (nv < 0 || ((nm & kMed) == 0 && m != 0)) {
+                        ended = true;
+                    }
+                    if (!ended) {
+                        p.lastvox = nv;
+                        p.med = nm;
+                    }
+                }
+            }
+        }
+        // ---- scatter when the free path is used up (:272-274) ----
+        if (!ended) {
+            while (p.lx <= 0.f) scatter(p, r, smedia[p.med & kMed]);
+            continue;
+        }
+
+        // ---- photon ends: detection (:276-278) and energy tallies (:765-770) ----
+        if (SAVEDET && (p.med & kDet) && p.ly <= a.tend) {
+            for (int d = 0; d < a.detnum; d++) {
+                const float4 dp = a.detpos[d];
+                float dx = dp.x - p.px, dy = dp.y - p.py, dz = dp.z - p.pz;
+                if (dx * dx + dy * dy + dz * dz < dp.w * dp.w) {
+                    uint32_t slot = atomicAdd(a.ndet, 1u);
+                    if (slot < a.maxdet) {
+                        float* row = a.detp + (uint64_t)slot * a.detlen;
+                        int c = 0;
+                        if (a.savedetflag & kDetID) row[c++] = (float)(d + 1);
+                        for (int m = 0; m < a.ppathlen; m++) row[c++] = sppath[m * bd + tid];
+                        if (a.savedetflag & kExitPos) { row[c++] = p.px; row[c++] = p.py; row[c++] = p.pz; }
+                        if (a.savedetflag & kExitDir) { row[c++] = p.vx; row[c++] = p.vy; row[c++] = p.vz; }
+                        a.detid[slot] = (uint32_t)pid;
+                        a.detw[slot] = p.pw;
+                    }
+                }
+            }
+        }
+        if (p.med & kMed) {
+            e_tot += (double)w0;
+            e_esc += (double)p.pw;
+            e_dep += ph_dep;
+        }
+        if (a.trace) {
+            TraceRec& t = a.trace[pid - a.first];
+            t.pos[0] = p.px; t.pos[1] = p.py; t.pos[2] = p.pz; t.pos[3] = p.pw;
+            t.vec[0] = p.vx; t.vec[1] = p.vy; t.vec[2] = p.vz; t.vec[3] = p.vw;
+            t.len[0] = p.lx; t.len[1] = p.ly; t.len[2] = p.lz; t.len[3] = p.lw;
+            t.mediaid = (int32_t)p.med; t.lastvoxelidx = p.lastvox; t.w0 = w0;
+        }
+        alive = false;
+    }
+
+    // ---- energies: wave sums, then one fp64 atomic per wave ----
+    e_esc = wave_sum(e_esc);
+    e_tot = wave_sum(e_tot);
+    e_dep = wave_sum(e_dep);
+    if (__lane_id() == 0) {
+        unsafeAtomicAdd(a.energy + 0, e_esc);
+        unsafeAtomicAdd(a.energy + 1, e_tot);
+        unsafeAtomicAdd(a.energy + 2, e_dep);
+    }
+}
+
+__global__ void zero_u64(unsigned long long* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = in[i], y;
+    switch (fn) {
+        case 0: y = libm::logf_host(x); break;
+        case 1: y = libm::sinf_host(x); break;
+        case 2: y = libm::cosf_host(x); break;
+        default: y = libm::acosf_host(x); break;
+    }
+    out[i] = y;
+}
+
+}  // namespace dev
+
+// ======================= host side =======================
+
+namespace {
+
+template <bool R, bool D, typename L>
+void launch_one(const dev::KernelArgs& ka, int blocks, int block, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((dev::photon_kernel<R, D, L>), dim3(blocks), dim3(block), lds, s, ka);
+}
+
+using LaunchFn = void (*)(const dev::KernelArgs&, int, int, size_t, hipStream_t);
+
+LaunchFn pick(bool reflect, bool savedet, bool u8) {
+    if (u8) {
+        if (reflect) return savedet ? launch_one<true, true, uint8_t> : launch_one<true, false, uint8_t>;
+        return savedet ? launch_one<false, true, uint8_t> : launch_one<false, false, uint8_t>;
+    }
+    if (reflect) return savedet ? launch_one<true, true, uint32_t> : launch_one<true, false, uint32_t>;
+    return savedet ? launch_one<false, true, uint32_t> : launch_one<false, false, uint32_t>;
+}
+
+const void* kernel_ptr(bool reflect, bool savedet, bool u8) {
+    if (u8) {
+        if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint8_t> : (const void*)dev::photon_kernel<true, false, uint8_t>;
+        return savedet ? (const void*)dev::photon_kernel<false, true, uint8_t> : (const void*)dev::photon_kernel<false, false, uint8_t>;
+    }
+    if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint32_t> : (const void*)dev::photon_kernel<true, false, uint32_t>;
+    return savedet ? (const void*)dev::photon_kernel<false, true, uint32_t> : (const void*)dev::photon_kernel<false, false, uint32_t>;
+}
+
+template <typename T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    HIP_OK(hipMalloc(&p, n * sizeof(T)));
+    return (T*)p;
+}
+
+}  // namespace
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+std::string device_name(int device) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return "";
+    return std::string(prop.name) + " (" + prop.gcnArchName + ")";
+}
+
+Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
+    if (device_count() <= 0) throw std::runtime_error("no HIP device available: the umcx GPU path needs an MI355X (gfx950)");
+    HIP_OK(hipSetDevice(lc.device));
+    hipStream_t s;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    stream_ = s;
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    ev0_ = e0;
+    ev1_ = e1;
+
+    const uint64_t nvox = (uint64_t)p.nx * p.ny * p.nz;
+    label_bytes_ = p.mediumnum <= 128 ? 1 : 4;
+    if (label_bytes_ == 1) {
+        std::vector<uint8_t> lab(nvox);
+        for (uint64_t i = 0; i < nvox; i++) {
+            uint32_t v = (uint32_t)p.labels[i];
+            lab[i] = (uint8_t)((v & kMedMask) | ((v & kDetMask) ? 0x80u : 0u));
+        }
+        d_labels_ = dalloc<uint8_t>(nvox);
+        HIP_OK(hipMemcpy(d_labels_, lab.data(), nvox, hipMemcpyHostToDevice));
+    } else {
+        d_labels_ = dalloc<uint32_t>(nvox);
+        HIP_OK(hipMemcpy(d_labels_, p.labels.data(), nvox * 4, hipMemcpyHostToDevice));
+    }
+    d_media_ = dalloc<float>(p.media.size());
+    HIP_OK(hipMemcpy(d_media_, p.media.data(), p.media.size() * 4, hipMemcpyHostToDevice));
+    d_detpos_ = dalloc<float>(std::max<size_t>(p.detpos.size(), 4));
+    if (!p.detpos.empty()) HIP_OK(hipMemcpy(d_detpos_, p.detpos.data(), p.detpos.size() * 4, hipMemcpyHostToDevice));
+    fluence_len_ = p.issavevol ? nvox * (uint64_t)p.maxgate : 0;
+    d_fluence_ = dalloc<float>(fluence_len_);
+    detcap_ = p.issavedet ? p.maxdetphotons : 0;
+    int cols = std::max(p.detp_columns(), 1);
+    d_detp_ = dalloc<float>((size_t)detcap_ * cols);
+    d_detid_ = dalloc<uint32_t>(detcap_);
+    d_detw_ = dalloc<float>(detcap_);
+    d_counters_ = dalloc<uint32_t>(4);
+    d_cursor_ = dalloc<unsigned long long>(2);
+    d_energy_ = dalloc<double>(4);
+
+    // launch shape: fill every CU with persistent waves
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, lc.device));
+    block_ = lc.block > 0 ? lc.block : (p.blocksize > 0 ? p.blocksize : 256);
+    block_ = std::max(64, std::min(1024, (block_ / 64) * 64));
+    size_t lds = (size_t)p.mediumnum * 16 + (size_t)p.ppath_columns() * block_ * 4;
+    while (lds > 64 * 1024 && block_ > 64) {
+        block_ /= 2;
+        lds = (size_t)p.mediumnum * 16 + (size_t)p.ppath_columns() * block_ * 4;
+    }
+    if (lds > 64 * 1024) throw std::runtime_error("too many media for the LDS staging of media and partial paths");
+    int per_cu = 0;
+    const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1), block_, lds));
+    if (per_cu < 1) per_cu = 1;
+    blocks_ = lc.blocks > 0 ? lc.blocks : prop.multiProcessorCount * per_cu;
+    if (p.threadnum > 0 && lc.blocks <= 0) blocks_ = std::max(1, p.threadnum / block_);
+    reset();
+}
+
+Session::~Session() {
+    (void)hipSetDevice(lc_.device);
+    for (void* ptr : {(void*)d_labels_, (void*)d_media_, (void*)d_detpos_, (void*)d_fluence_, (void*)d_detp_,
+                      (void*)d_detid_, (void*)d_detw_, (void*)d_counters_, (void*)d_cursor_, (void*)d_energy_,
+                      (void*)d_trace_})
+        if (ptr) (void)hipFree(ptr);
+    if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
+    if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
+    if (stream_) (void)hipStreamDestroy((hipStream_t)stream_);
+}
+
+void Session::reset() {
+    HIP_OK(hipSetDevice(lc_.device));
+    hipStream_t s = (hipStream_t)stream_;
+    if (fluence_len_) HIP_OK(hipMemsetAsync(d_fluence_, 0, fluence_len_ * 4, s));
+    HIP_OK(hipMemsetAsync(d_counters_, 0, 16, s));
+    HIP_OK(hipMemsetAsync(d_energy_, 0, 32, s));
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+RunStats Session::run(uint64_t first, uint64_t count) {
+    HIP_OK(hipSetDevice(lc_.device));
+    hipStream_t s = (hipStream_t)stream_;
+    const Problem& p = prob_;
+    RunStats st;
+    st.photons = count;
+    last_count_ = count;
+    if (lc_.trace) {
+        if (trace_cap_ < count) {
+            if (d_trace_) HIP_OK(hipFree(d_trace_));
+            d_trace_ = dalloc<TraceRec>(count);
+            trace_cap_ = count;
+        }
+        HIP_OK(hipMemsetAsync(d_trace_, 0, count * sizeof(TraceRec), s));
+    }
+    if (count == 0) return st;
+    dev::KernelArgs ka;
+    memset(&ka, 0, sizeof ka);
+    ka.nx = p.nx; ka.ny = p.ny; ka.nz = p.nz; ka.nyz = p.ny * p.nz;
+    ka.nxyz = (uint64_t)p.nx * p.ny * p.nz;
+    ka.maxgate = p.maxgate; ka.outputtype = p.outputtype; ka.savedetflag = p.savedetflag;
+    ka.mediumnum = p.mediumnum; ka.detnum = p.issavedet ? p.detnum : 0; ka.srctype = p.srctype;
+    ka.issavevol = p.issavevol; ka.ppathlen = p.ppath_columns(); ka.detlen = p.detp_columns();
+    ka.tstart = p.tstart; ka.tend = p.tend; ka.rtstep = p.rtstep;
+    ka.srcpos = make_float4(p.srcpos[0], p.srcpos[1], p.srcpos[2], p.srcpos[3]);
+    ka.srcdir = make_float4(p.srcdir[0], p.srcdir[1], p.srcdir[2], p.srcdir[3]);
+    ka.srcp1 = make_float4(p.srcparam1[0], p.srcparam1[1], p.srcparam1[2], p.srcparam1[3]);
+    ka.srcp2 = make_float4(p.srcparam2[0], p.srcparam2[1], p.srcparam2[2], p.srcparam2[3]);
+    for (int i = 0; i < 4; i++) ka.seeds[i] = p.seeds[i];
+    ka.first = first; ka.count = count;
+    ka.labels = d_labels_;
+    ka.media = (const float4*)d_media_;
+    ka.detpos = (const float4*)d_detpos_;
+    ka.fluence = d_fluence_;
+    ka.detp = d_detp_; ka.detid = d_detid_; ka.detw = d_detw_; ka.ndet = d_counters_; ka.maxdet = detcap_;
+    ka.energy = d_energy_;
+    ka.cursor = d_cursor_;
+    ka.trace = lc_.trace ? d_trace_ : nullptr;
+
+    const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
+    size_t lds = (size_t)p.mediumnum * 16 + (size_t)ka.ppathlen * block_ * 4;
+    hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
+    pick(reflect, savedet, label_bytes_ == 1)(ka, blocks_, block_, lds, s);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
+    HIP_OK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, (hipEvent_t)ev0_, (hipEvent_t)ev1_));
+    st.kernel_ms = ms;
+    return st;
+}
+
+void Session::fetch_fluence(float* out) const {
+    HIP_OK(hipSetDevice(lc_.device));
+    if (fluence_len_) HIP_OK(hipMemcpy(out, d_fluence_, fluence_len_ * 4, hipMemcpyDeviceToHost));
+}
+
+uint32_t Session::detected() const {
+    HIP_OK(hipSetDevice(lc_.device));
+    uint32_t n = 0;
+    HIP_OK(hipMemcpy(&n, d_counters_, 4, hipMemcpyDeviceToHost));
+    return n;
+}
+
+uint32_t Session::saved() const { return std::min(detected(), detcap_); }
+
+void Session::fetch_detp(float* rows, uint32_t* ids, float* weights) const {
+    HIP_OK(hipSetDevice(lc_.device));
+    uint32_t n = saved();
+    int cols = prob_.detp_columns();
+    if (rows && n && cols) HIP_OK(hipMemcpy(rows, d_detp_, (size_t)n * cols * 4, hipMemcpyDeviceToHost));
+    if (ids && n) HIP_OK(hipMemcpy(ids, d_detid_, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (weights && n) HIP_OK(hipMemcpy(weights, d_detw_, (size_t)n * 4, hipMemcpyDeviceToHost));
+}
+
+void Session::energies(double out[3]) const {
+    HIP_OK(hipSetDevice(lc_.device));
+    HIP_OK(hipMemcpy(out, d_energy_, 24, hipMemcpyDeviceToHost));
+}
+
+void Session::fetch_trace(TraceRec* out, uint64_t n) const {
+    HIP_OK(hipSetDevice(lc_.device));
+    if (!d_trace_) throw std::runtime_error("trace was not enabled for this session");
+    n = std::min(n, last_count_);
+    HIP_OK(hipMemcpy(out, d_trace_, n * sizeof(TraceRec), hipMemcpyDeviceToHost));
+}
+
+void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device) {
+    HIP_OK(hipSetDevice(device));
+    float *din = dalloc<float>(n), *dout = dalloc<float>(n);
+    HIP_OK(hipMemcpy(din, in, n * 4, hipMemcpyHostToDevice));
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(dev::libm_eval_kernel, dim3(blocks), dim3(256), 0, 0, fn, din, dout, n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+    (void)hipFree(din);
+    (void)hipFree(dout);
+}
+
+}  // namespace umcx

@@ -1,0 +1,94 @@
+// runtime.h -- device-side state of one simulation and the launches that fill it.
+//
+// A Session uploads one Problem to the GPU once (labels as uint8 when the media fit in 7 bits,
+// media and detectors, zeroed tallies) and then runs photon ranges [first, first+count) of
+// the MCX_kernel loop (src/umcx.cpp:752-775) with the hand-written gfx950 kernel. Tallies
+// accumulate across runs until reset(), so a photon range can be split over calls.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "config.h"
+
+namespace umcx {
+
+struct LaunchConfig {
+    int device = 0;          // HIP device ordinal
+    int block = 0;           // threads per workgroup (0: tuned default)
+    int blocks = 0;          // persistent workgroups (0: fill the GPU)
+    int tally_mode = -1;     // -1 tuned, 0 global atomics, 1 LDS-cached tile
+    bool trace = false;      // keep each photon's final state (parity tests; small runs only)
+};
+
+// Final state of one photon, for per-photon parity checks.
+struct TraceRec {
+    float pos[4], vec[4], len[4];
+    int32_t mediaid, lastvoxelidx;
+    float w0;
+    uint32_t pad;
+};
+
+struct RunStats {
+    double kernel_ms = 0.0;  // HIP-event time of the photon kernel(s) on the session stream
+    uint64_t photons = 0;
+};
+
+class Session {
+public:
+    Session(const Problem& p, const LaunchConfig& lc);
+    ~Session();
+    Session(const Session&) = delete;
+    Session& operator=(const Session&) = delete;
+
+    void reset();                                        // zero fluence, detections, energies
+    RunStats run(uint64_t first, uint64_t count);        // launch and wait
+    // host copies of the raw (unnormalised) tallies
+    void fetch_fluence(float* out) const;                // maxgate*nx*ny*nz, [t][x][y][z]
+    uint32_t detected() const;                           // all detections, incl. dropped
+    uint32_t saved() const;                              // rows kept (<= maxdetphotons)
+    void fetch_detp(float* rows, uint32_t* ids, float* weights) const;
+    void energies(double out[3]) const;                  // escaped, launched, deposited (fp64)
+    void fetch_trace(TraceRec* out, uint64_t n) const;   // of the last run
+    uint64_t fluence_len() const { return fluence_len_; }
+    const Problem& problem() const { return prob_; }
+    int block() const { return block_; }
+    int blocks() const { return blocks_; }
+    int tally_mode() const { return tally_mode_; }
+    void* stream() const { return stream_; }
+    // device pointers, for callers that keep results on the GPU
+    float* device_fluence() const { return d_fluence_; }
+
+private:
+    Problem prob_;
+    LaunchConfig lc_;
+    int block_ = 256, blocks_ = 0, tally_mode_ = 0;
+    int label_bytes_ = 1;
+    uint64_t fluence_len_ = 0;
+    void* stream_ = nullptr;
+    void* ev0_ = nullptr;
+    void* ev1_ = nullptr;
+    void* d_labels_ = nullptr;
+    float* d_media_ = nullptr;
+    float* d_detpos_ = nullptr;
+    float* d_fluence_ = nullptr;
+    float* d_detp_ = nullptr;
+    uint32_t* d_detid_ = nullptr;
+    float* d_detw_ = nullptr;
+    uint32_t* d_counters_ = nullptr;     // [0] detections
+    unsigned long long* d_cursor_ = nullptr;
+    double* d_energy_ = nullptr;
+    TraceRec* d_trace_ = nullptr;
+    uint64_t trace_cap_ = 0;
+    uint64_t last_count_ = 0;
+    uint32_t detcap_ = 0;
+};
+
+// Replica checks: evaluate the device libm replicas on n inputs (fn: 0 logf, 1 sinf, 2 cosf,
+// 3 acosf) on the GPU.
+void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device);
+
+int device_count();
+std::string device_name(int device);
+
+}  // namespace umcx
