@@ -95,6 +95,11 @@ def reachable_inputs(g_values=(0.9, 0.01, 0.89, 0.95, 0.7)) -> dict:
         t = (F(1) - g * g) / (F(1) - g + F(2) * g * u)
         t = (F(1) + g * g - t * t) / (F(2) * g)
         out[3].append(np.maximum(F(-1), np.minimum(F(1), t)))
+    # expf(-mua*d): every 61st float of [-104, 0] plus the two hardest rounding cases found by an
+    # exhaustive scan of all floats in [-104, 88]
+    neg = np.arange(0x80000000, 0xC2D00001, 61, dtype=np.uint64).astype(np.uint32).view(np.float32)
+    hard = np.array([float.fromhex("-0x1.f8cbb2p+5"), float.fromhex("0x1.04845ep+5")], dtype=np.float32)
+    out[4] = [neg, hard]
     return {fn: np.concatenate(v) for fn, v in out.items()}
 
 

@@ -2,9 +2,8 @@
 reference's golden vectors.
 
 Bars:
-* per photon: position, direction, scatter count, time of flight, path and medium bitwise equal
-  to the oracle; the weight within 1e-5 relative (it decays through expf, whose last bit may
-  differ between the device and glibc; weights never steer photons);
+* per photon: position, direction, scatter count, time of flight, path, medium and weight bitwise
+  equal to the oracle;
 * detected-photon rows: bitwise equal to the reference's rows as a multiset;
 * fluence: relative L2 <= 1e-4 against the reference's volume;
 * energy balance from fp64 tallies: |deposited + escaped - launched| <= 1e-6 launched;
@@ -64,8 +63,8 @@ def test_per_photon_state_matches_oracle(name):
     assert np.array_equal(t["mediaid"], o["mediaid"])
     assert np.array_equal(t["lastvoxelidx"], o["lastvoxelidx"])
     assert np.array_equal(exact(t["w0"]), exact(o["w0"]))
-    np.testing.assert_allclose(t["pos"][:, 3], o["pos"][:, 3], rtol=1e-5, atol=1e-37)
-    np.testing.assert_allclose(t["len"][:, 3], o["len"][:, 3], rtol=1e-5, atol=1e-37)
+    assert np.array_equal(exact(t["pos"][:, 3]), exact(o["pos"][:, 3]))
+    assert np.array_equal(exact(t["len"][:, 3]), exact(o["len"][:, 3]))
 
 
 @pytest.mark.parametrize("name", DETP_CASES)
@@ -105,7 +104,13 @@ def test_fluence_and_energy_match_reference(name):
         res = sim.result(normalize=normalize)
     assert res.energy_total == pytest.approx(float(g["energy"]), abs=0.006)
     assert abs(res.absorbed_percent - float(g["absorbed"])) <= 1e-3
-    assert abs(res.energy_deposit + res.energy_escape - res.energy_total) <= 1e-6 * res.energy_total
+    if prob["srctype"] == 5:
+        # fourier sources launched outside the grid: skip() restarts every photon at weight 1
+        # (src/umcx.cpp:353) while the launch weight is what energytotal counts (:762, :767)
+        launched = float(int(float(g["cfg"]["Session"]["Photons"])))
+    else:
+        launched = res.energy_total
+    assert abs(res.energy_deposit + res.energy_escape - launched) <= 1e-6 * launched
     f = res.fluence.reshape(-1)
     if "fluence" in g:
         ref = g["fluence"]
@@ -141,7 +146,8 @@ def test_command_line_outputs_are_byte_compatible(name, tmp_path):
         raw = (tmp_path / f"{name}_detp.jdb").read_bytes()
         assert len(raw) == int(g["detp_size"])
         dh = bytes(g["detp_head"])
-        assert raw[:len(dh)] == dh
+        payload = len(raw) - int(g["detp"].size) * 4      # rows follow the header; their order varies
+        assert raw[:payload] == dh[:payload]
         info, rows = bjdata.load_detp(str(tmp_path / f"{name}_detp.jdb"))
         assert np.array_equal(sorted_rows(rows).view(np.uint32), g["detp"].view(np.uint32))
 
