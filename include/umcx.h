@@ -92,7 +92,7 @@ int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels);
 int umcx_config_from_argv(char** argv, int argn, char** json_out);
 /* glibc srand(seed)+4*rand() words used to seed every photon (src/umcx.cpp:725-727). */
 int umcx_seed_words(int32_t seed, uint32_t out[4]);
-/* Evaluate the libm replicas (fn 0 logf, 1 sinf, 2 cosf, 3 acosf) on the host CPU. */
+/* Evaluate the libm replicas (fn 0 logf, 1 sinf, 2 cosf, 3 acosf, 4 expf) on the host CPU. */
 int umcx_libm_host(int32_t fn, const float* in, float* out, uint64_t n);
 /* Same on the GPU (device ordinal), for bitwise checks of the device build. */
 int umcx_libm_device(int32_t fn, const float* in, float* out, uint64_t n, int32_t device);

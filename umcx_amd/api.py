@@ -209,7 +209,7 @@ def seed_words(seed: int) -> list[int]:
 
 
 def libm_eval(fn: int, x: np.ndarray, device: int | None = None) -> np.ndarray:
-    """Evaluate a libm replica (0 logf, 1 sinf, 2 cosf, 3 acosf) on the CPU or the GPU."""
+    """Evaluate a libm replica (0 logf, 1 sinf, 2 cosf, 3 acosf, 4 expf) on the CPU or the GPU."""
     x = np.ascontiguousarray(x, dtype=np.float32)
     out = np.empty_like(x)
     if device is None:

@@ -1,21 +1,22 @@
 // glibc_libm.h -- bit-exact device replicas of the host C library's single-precision
-// logf, sinf, cosf and acosf, plus nextafterf.
+// logf, sinf, cosf, acosf and expf, plus nextafterf.
 //
 // Why: umcx's photon trajectories go through libm (src/umcx.cpp:213 logf, :437-438 sinf/cosf,
 // :232,:372,:374 acosf, :31 nextafterf). glibc 2.35 is not correctly rounded on 0.7-7.7% of the
 // arguments the photon loop can reach, so a correctly rounded GPU libm would send almost every
 // photon down a different path. These functions reproduce the host library instead, from the
 // published designs it implements:
-//   * logf, sinf, cosf: the ARM optimized-routines algorithms (Szabolcs Nagy, 2017-2018) that
-//     glibc uses since 2.28 -- a 16-entry (invc, log c) table with a cubic in double for logf;
-//     double range reduction by pi/2 and even/odd polynomials for sinf/cosf. The x86-64 glibc
-//     selects its FMA-compiled variants on FMA-capable hosts; the fused forms are used here.
+//   * logf, sinf, cosf, expf: the ARM optimized-routines algorithms (Szabolcs Nagy, 2017-2018)
+//     that glibc uses since 2.28 -- a 16-entry (invc, log c) table with a cubic in double for
+//     logf; double range reduction by pi/2 and even/odd polynomials for sinf/cosf; a 32-entry
+//     2^(i/32) table and a cubic for expf. The x86-64 glibc selects its FMA-compiled variants
+//     on FMA-capable hosts; the fused forms are used here.
 //   * acosf: the fdlibm single-precision algorithm (Sun Microsystems, 1993) that glibc's
 //     sysdeps/ieee754/flt-32 keeps: rational approximation of asin on [0, 0.5] and the
 //     half-angle reduction above it, in float.
-// Verified against the host libm on every float of logf (0,1], sinf/cosf [0,8) and acosf
-// [-1,1] (0 mismatches, FMA and non-FMA orders alike), and on the GPU over every argument the
-// photon loop can form (tests/test_libm_replica.py).
+// Verified against the host libm on every float of logf (0,1], sinf/cosf [0,8), acosf [-1,1]
+// and expf [-104,88] (0 mismatches), and on the GPU over every argument the photon loop can form
+// (tests/test_host_contract.py, tests/test_gpu_parity.py).
 //
 // Only the argument ranges the photon loop uses are implemented: logf for positive normal
 // inputs (u + FLT_EPSILON >= 2^-23), sinf/cosf for |x| < 120, acosf on [-1, 1].
@@ -188,6 +189,62 @@ UMCX_HD float acosf_host(float x) {
     float r = acos_rational(z);
     float w = r * s + c;
     return 2.0f * (df + w);
+}
+
+// ---- expf: 2^(k/32) * 2^(r/32) with a cubic for 2^(r/32), in double ----
+// kExp2Tab[i] = bits(2^(i/32) rounded to double) - (i << 47): the integer k/32 is added to the
+// exponent field afterwards. Values computed from 2^(i/32) at 80 digits (not copied from any
+// binary). The FMA build of the host library fuses InvLn2N*x into both of its uses.
+#if defined(__HIPCC__)
+__device__ __constant__
+#endif
+static const uint64_t kExp2Tab[32] = {
+    0x3ff0000000000000ull, 0x3fefd9b0d3158574ull, 0x3fefb5586cf9890full, 0x3fef9301d0125b51ull,
+    0x3fef72b83c7d517bull, 0x3fef54873168b9aaull, 0x3fef387a6e756238ull, 0x3fef1e9df51fdee1ull,
+    0x3fef06fe0a31b715ull, 0x3feef1a7373aa9cbull, 0x3feedea64c123422ull, 0x3feece086061892dull,
+    0x3feebfdad5362a27ull, 0x3feeb42b569d4f82ull, 0x3feeab07dd485429ull, 0x3feea47eb03a5585ull,
+    0x3feea09e667f3bcdull, 0x3fee9f75e8ec5f74ull, 0x3feea11473eb0187ull, 0x3feea589994cce13ull,
+    0x3feeace5422aa0dbull, 0x3feeb737b0cdc5e5ull, 0x3feec49182a3f090ull, 0x3feed503b23e255dull,
+    0x3feee89f995ad3adull, 0x3feeff76f2fb5e47ull, 0x3fef199bdd85529cull, 0x3fef3720dcef9069ull,
+    0x3fef5818dcfba487ull, 0x3fef7c97337b9b5full, 0x3fefa4afa2a490daull, 0x3fefd0765b6e4540ull,
+};
+
+UMCX_HD double u642d(uint64_t u) {
+#if defined(__HIPCC__)
+    return __builtin_bit_cast(double, u);
+#else
+    double d; memcpy(&d, &u, 8); return d;
+#endif
+}
+UMCX_HD uint64_t d2u64(double d) {
+#if defined(__HIPCC__)
+    return __builtin_bit_cast(uint64_t, d);
+#else
+    uint64_t u; memcpy(&u, &d, 8); return u;
+#endif
+}
+
+// exp(x) for x in [-103.97, 88.72] (the photon loop only passes -mua*d <= 0); outside that
+// range it saturates to 0 / +inf like the host library.
+UMCX_HD float expf_host(float x) {
+    const double inv_ln2_n = 0x1.71547652b82fep+0 * 32, shift = 0x1.8p+52;
+    const double c0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32, c1 = 0x1.ebfce50fac4f3p-3 / 32 / 32,
+                 c2 = 0x1.62e42ff0c52d6p-1 / 32;
+    if (x != x) return x + x;
+    if (x < -0x1.9fe368p6f) return 0.0f;
+    if (x > 0x1.62e42ep6f) return u2f(0x7f800000u);
+    double xd = (double)x;
+    double kd = dfma(inv_ln2_n, xd, shift);
+    uint64_t ki = d2u64(kd);
+    kd -= shift;
+    double r = dfma(inv_ln2_n, xd, -kd);
+    uint64_t t = kExp2Tab[ki % 32] + (ki << 47);
+    double s = u642d(t);
+    double z = dfma(c0, r, c1);
+    double r2 = r * r;
+    double y = dfma(c2, r, 1.0);
+    y = dfma(z, r2, y);
+    return (float)(y * s);
 }
 
 // ---- nextafterf(x, y) by bit stepping (IEEE 754 semantics, denormals kept) ----

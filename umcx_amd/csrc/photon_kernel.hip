@@ -8,9 +8,9 @@
 //
 // Arithmetic follows the reference's FP32 operation order exactly (no contraction: built with
 // -ffp-contract=off; division and sqrt correctly rounded; denormals kept), and the
-// trajectory-affecting libm calls use the bit-exact replicas of glibc_libm.h. The photon weight
-// only decays through expf (src/umcx.cpp:326) and never steers a photon, so expf is the
-// device's own.
+// libm calls use the bit-exact replicas of glibc_libm.h -- logf/sinf/cosf/acosf steer the photon;
+// expf (src/umcx.cpp:326) only decays its weight, and is replicated too so that weights and
+// fluence deposits are also bitwise equal to the reference's, up to atomic summation order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -219,7 +219,7 @@ __device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a)
     p.px = p.px + d * p.vx;
     p.py = p.py + d * p.vy;
     p.pz = p.pz + d * p.vz;
-    p.pw = p.pw * expf(-mua * d);
+    p.pw = p.pw * libm::expf_host(-mua * d);
     p.lx -= used;
     p.ly += d * n * kOneOverC0;
     p.lz += d;
@@ -483,7 +483,8 @@ __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n
         case 0: y = libm::logf_host(x); break;
         case 1: y = libm::sinf_host(x); break;
         case 2: y = libm::cosf_host(x); break;
-        default: y = libm::acosf_host(x); break;
+        case 3: y = libm::acosf_host(x); break;
+        default: y = libm::expf_host(x); break;
     }
     out[i] = y;
 }

@@ -85,7 +85,7 @@ private:
 };
 
 // Replica checks: evaluate the device libm replicas on n inputs (fn: 0 logf, 1 sinf, 2 cosf,
-// 3 acosf) on the GPU.
+// 3 acosf, 4 expf) on the GPU.
 void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device);
 
 int device_count();
