@@ -79,6 +79,9 @@ int umcx_session_detp(const umcx_session* s, float* rows, uint32_t* ids, float* 
 int umcx_session_energies(const umcx_session* s, double out[3]);
 /* Per-photon final state of the last run (64 bytes each), when launched with trace = 1. */
 int umcx_session_trace(const umcx_session* s, void* out, uint64_t n);
+/* The LDS tally tile in use, {x0, y0, z0, gate0, nx, ny, nz, ngates} (zeros when none).
+ * No counterpart in the reference. */
+int umcx_session_tally_box(const umcx_session* s, int32_t box[8]);
 /* Output normaliser of src/umcx.cpp:820 for a given launched energy. */
 float umcx_session_normalizer(const umcx_session* s, double energytotal);
 void umcx_session_free(umcx_session* s);

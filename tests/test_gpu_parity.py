@@ -94,12 +94,17 @@ def test_detector_overflow_keeps_genuine_rows():
     assert all(r.tobytes() in allrows for r in rows)
 
 
+@pytest.mark.parametrize("tally", [0, 1])
 @pytest.mark.parametrize("name", FLUENCE_CASES)
-def test_fluence_and_energy_match_reference(name):
+def test_fluence_and_energy_match_reference(name, tally):
+    """tally 0: every deposit is a global atomic; tally 1: deposits inside the pilot-chosen box
+    accumulate in LDS first."""
     g, prob = O.golden_problem(name)
-    say(f"fluence parity {name}")
-    with U.Simulation(g["cfg"]) as sim:
+    say(f"fluence parity {name} tally {tally}")
+    with U.Simulation(g["cfg"], tally_mode=tally) as sim:
         sim.run()
+        if tally == 1 and prob["issavevol"]:
+            assert sim.tally_box() is not None
         normalize = bool(prob["isnormalized"]) and "-U" not in g["argv"]
         res = sim.result(normalize=normalize)
     assert res.energy_total == pytest.approx(float(g["energy"]), abs=0.006)

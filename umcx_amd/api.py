@@ -112,6 +112,12 @@ class Simulation:
         check(lib().umcx_session_fluence(self._h, out.ctypes.data, out.size))
         return out.reshape(nt, nx, ny, nz)
 
+    def tally_box(self):
+        """The LDS tally tile chosen for this session, {x0,y0,z0,g0,sx,sy,sz,sg}, or None."""
+        b = (C.c_int32 * 8)()
+        check(lib().umcx_session_tally_box(self._h, b))
+        return list(b) if b[4] > 0 else None
+
     def fluence_device_ptr(self) -> int:
         p = C.c_void_p()
         check(lib().umcx_session_fluence_device(self._h, C.byref(p)))

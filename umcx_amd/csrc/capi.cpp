@@ -237,6 +237,13 @@ int umcx_session_trace(const umcx_session* s, void* out, uint64_t n) {
     return guarded([&]() -> int { s->s->fetch_trace((umcx::TraceRec*)out, n); return 0; });
 }
 
+int umcx_session_tally_box(const umcx_session* s, int32_t box[8]) {
+    return guarded([&]() -> int {
+        for (int i = 0; i < 8; i++) box[i] = s->s->tile()[i];
+        return 0;
+    });
+}
+
 float umcx_session_normalizer(const umcx_session* s, double energytotal) {
     return normalizer_of(s->s->problem(), energytotal);
 }

@@ -58,6 +58,10 @@ struct KernelArgs {
     double* energy;
     unsigned long long* cursor;
     TraceRec* trace;
+    // LDS tally tile: box [tx0,tx0+tsx) x [ty0,..) x [tz0,..) over gates [tg0,tg0+tsg)
+    int32_t tx0, ty0, tz0, tg0;
+    uint32_t tsx, tsy, tsz, tsg, tcells, tile_n;
+    int32_t count_mode;      // 1: tally the number of deposits instead of their values
 };
 
 // ---------------- xorshift128+ (src/umcx.cpp:186-215) ----------------
@@ -89,6 +93,7 @@ struct Photon {
     int ix, iy, iz, iw;        // voxel (short semantics) and crossing axis
     int lastvox;
     uint32_t med;
+    int tvox;                  // offset of lastvox inside the LDS tally tile, -1 outside it
 };
 
 template <typename LabelT>
@@ -109,6 +114,11 @@ __device__ __forceinline__ int to_short(float x) { return (int)(short)(int)x; }
 __device__ __forceinline__ int vox_index(const KernelArgs& a, int ix, int iy, int iz) {
     if ((unsigned)ix >= a.nx || (unsigned)iy >= a.ny || (unsigned)iz >= a.nz) return -1;
     return ix * (int)a.nyz + iy * (int)a.nz + iz;
+}
+
+__device__ __forceinline__ int tile_local(const KernelArgs& a, int ix, int iy, int iz) {
+    const unsigned dx = (unsigned)(ix - a.tx0), dy = (unsigned)(iy - a.ty0), dz = (unsigned)(iz - a.tz0);
+    return (dx < a.tsx && dy < a.tsy && dz < a.tsz) ? (int)((dx * a.tsy + dy) * a.tsz + dz) : -1;
 }
 
 // rotatevector (:418-428)
@@ -327,11 +337,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 template <bool REFLECT, bool SAVEDET, typename LabelT>
 __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
+    // LDS: media | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
     float4* smedia = lds4;
-    float* sppath = (float*)(lds4 + a.mediumnum);
+    float* stile = (float*)(lds4 + a.mediumnum);
+    float* sppath = stile + a.tile_n;
     const int tid = threadIdx.x, bd = blockDim.x;
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
+    for (int i = tid; i < (int)a.tile_n; i += bd) stile[i] = 0.f;
     if (SAVEDET)
         for (int i = tid; i < a.ppathlen * bd; i += bd) sppath[i] = 0.f;
     __syncthreads();
@@ -377,6 +390,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 }
                 continue;
             }
+            p.tvox = tile_local(a, p.ix, p.iy, p.iz);
             p.med = load_label<LabelT>(a.labels, p.lastvox);                                         // :269
             p.lx = rng_scatlen(r);                                                                   // :270
             alive = true;
@@ -395,8 +409,14 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                     float dw = p.lw - p.pw;
                     float val = energy_out ? dw : ((prop.x < kEps) ? (p.lw * p.lz) : dw / prop.x);
                     ph_dep += (double)dw;
+                    if (a.count_mode) val = 1.f;
                     if (tshift >= 0) {
-                        unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
+                        const unsigned g = (unsigned)(tshift - a.tg0);
+                        if (p.tvox >= 0 && g < a.tsg)
+                            __hip_atomic_fetch_add(stile + g * a.tcells + (unsigned)p.tvox, val, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        else
+                            unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
                     }
                     p.lw = p.pw;
                     p.lz = 0.f;
@@ -412,6 +432,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                         ended = true;
                     }
                     if (!ended) {
+                        if (nv != p.lastvox) p.tvox = tile_local(a, p.ix, p.iy, p.iz);
                         p.lastvox = nv;
                         p.med = nm;
                     }
@@ -457,6 +478,22 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             t.mediaid = (int32_t)p.med; t.lastvoxelidx = p.lastvox; t.w0 = w0;
         }
         alive = false;
+    }
+
+    // ---- flush the LDS tally tile ----
+    if (a.tile_n) {
+        __syncthreads();
+        const uint32_t plane = a.tsy * a.tsz;
+        for (uint32_t i = tid; i < a.tile_n; i += bd) {
+            const float v = stile[i];
+            if (v != 0.f) {
+                const uint32_t g = i / a.tcells, c = i - g * a.tcells;
+                const uint32_t dx = c / plane, rem = c - dx * plane, dy = rem / a.tsz, dz = rem - dy * a.tsz;
+                const uint64_t idx = (uint64_t)(a.tg0 + g) * a.nxyz + (uint64_t)(a.tx0 + dx) * a.nyz +
+                                     (uint64_t)(a.ty0 + dy) * a.nz + (uint64_t)(a.tz0 + dz);
+                unsafeAtomicAdd(a.fluence + idx, v);
+            }
+        }
     }
 
     // ---- energies: wave sums, then one fp64 atomic per wave ----
@@ -588,16 +625,23 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     HIP_OK(hipGetDeviceProperties(&prop, lc.device));
     block_ = lc.block > 0 ? lc.block : (p.blocksize > 0 ? p.blocksize : 256);
     block_ = std::max(64, std::min(1024, (block_ / 64) * 64));
-    size_t lds = (size_t)p.mediumnum * 16 + (size_t)p.ppath_columns() * block_ * 4;
-    while (lds > 64 * 1024 && block_ > 64) {
+    size_t lds = lds_bytes(0);
+    const size_t lds_wg_max = std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
+    while (lds > lds_wg_max && block_ > 64) {
         block_ /= 2;
-        lds = (size_t)p.mediumnum * 16 + (size_t)p.ppath_columns() * block_ * 4;
+        lds = lds_bytes(0);
     }
-    if (lds > 64 * 1024) throw std::runtime_error("too many media for the LDS staging of media and partial paths");
+    if (lds > lds_wg_max) throw std::runtime_error("too many media for the LDS staging of media and partial paths");
     int per_cu = 0;
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
     HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1), block_, lds));
     if (per_cu < 1) per_cu = 1;
+    // LDS tally tile: share the CU's LDS between the workgroups the register budget lets in
+    tally_mode_ = lc.tally_mode;
+    const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 160 * 1024;
+    size_t per_wg = std::min(lds_cu / (size_t)per_cu, lds_wg_max);
+    tile_budget_ = per_wg > lds ? std::min<uint64_t>((per_wg - lds) / 4, 8192) : 0;
+    if (tally_mode_ == 1 && tile_budget_ < 64) tile_budget_ = 64;
     blocks_ = lc.blocks > 0 ? lc.blocks : prop.multiProcessorCount * per_cu;
     if (p.threadnum > 0 && lc.blocks <= 0) blocks_ = std::max(1, p.threadnum / block_);
     reset();
@@ -623,23 +667,61 @@ void Session::reset() {
     HIP_OK(hipStreamSynchronize(s));
 }
 
-RunStats Session::run(uint64_t first, uint64_t count) {
-    HIP_OK(hipSetDevice(lc_.device));
-    hipStream_t s = (hipStream_t)stream_;
-    const Problem& p = prob_;
-    RunStats st;
-    st.photons = count;
-    last_count_ = count;
-    if (lc_.trace) {
-        if (trace_cap_ < count) {
-            if (d_trace_) HIP_OK(hipFree(d_trace_));
-            d_trace_ = dalloc<TraceRec>(count);
-            trace_cap_ = count;
-        }
-        HIP_OK(hipMemsetAsync(d_trace_, 0, count * sizeof(TraceRec), s));
+namespace {
+
+// Greedy growth of an axis-aligned (gate, x, y, z) box around the densest cell of a count grid,
+// adding at each step the face slab with the most deposits per added cell, while the box holds at
+// most `budget` cells. Returns {x0, y0, z0, g0, sx, sy, sz, sg}.
+std::vector<int> choose_box(const std::vector<float>& cnt, int nx, int ny, int nz, int nt, uint64_t budget) {
+    const uint64_t nyz = (uint64_t)ny * nz, nxyz = nyz * nx;
+    auto at = [&](int g, int x, int y, int z) -> double { return cnt[(uint64_t)g * nxyz + (uint64_t)x * nyz + (uint64_t)y * nz + z]; };
+    uint64_t best = 0;
+    for (uint64_t i = 1; i < cnt.size(); i++) if (cnt[i] > cnt[best]) best = i;
+    int lo[4], hi[4];     // g, x, y, z half-open
+    lo[0] = (int)(best / nxyz); uint64_t r = best % nxyz;
+    lo[1] = (int)(r / nyz); r %= nyz;
+    lo[2] = (int)(r / nz); lo[3] = (int)(r % nz);
+    for (int d = 0; d < 4; d++) hi[d] = lo[d] + 1;
+    const int ext[4] = {nt, nx, ny, nz};
+    auto cells = [&](const int* l, const int* h) { uint64_t c = 1; for (int d = 0; d < 4; d++) c *= (uint64_t)(h[d] - l[d]); return c; };
+    auto slab = [&](int d, int at_idx) {
+        int l[4], h[4];
+        for (int k = 0; k < 4; k++) { l[k] = lo[k]; h[k] = hi[k]; }
+        l[d] = at_idx; h[d] = at_idx + 1;
+        double sum = 0;
+        for (int g = l[0]; g < h[0]; g++)
+            for (int x = l[1]; x < h[1]; x++)
+                for (int y = l[2]; y < h[2]; y++)
+                    for (int z = l[3]; z < h[3]; z++) sum += at(g, x, y, z);
+        return sum;
+    };
+    while (true) {
+        int bd = -1, bside = 0;
+        double bscore = 0;
+        for (int d = 0; d < 4; d++)
+            for (int side = 0; side < 2; side++) {
+                int idx = side ? hi[d] : lo[d] - 1;
+                if (idx < 0 || idx >= ext[d]) continue;
+                int l[4], h[4];
+                for (int k = 0; k < 4; k++) { l[k] = lo[k]; h[k] = hi[k]; }
+                if (side) h[d]++; else l[d]--;
+                uint64_t nc = cells(l, h);
+                if (nc > budget) continue;
+                double add = slab(d, idx);
+                double score = add / (double)(nc - cells(lo, hi));
+                if (add > 0 && score > bscore) { bscore = score; bd = d; bside = side; }
+            }
+        if (bd < 0) break;
+        if (bside) hi[bd]++; else lo[bd]--;
     }
-    if (count == 0) return st;
-    dev::KernelArgs ka;
+    return {lo[1], lo[2], lo[3], lo[0], hi[1] - lo[1], hi[2] - lo[2], hi[3] - lo[3], hi[0] - lo[0]};
+}
+
+}  // namespace
+
+void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
+    const Problem& p = prob_;
+    dev::KernelArgs& ka = *(dev::KernelArgs*)out;
     memset(&ka, 0, sizeof ka);
     ka.nx = p.nx; ka.ny = p.ny; ka.nz = p.nz; ka.nyz = p.ny * p.nz;
     ka.nxyz = (uint64_t)p.nx * p.ny * p.nz;
@@ -661,9 +743,69 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     ka.energy = d_energy_;
     ka.cursor = d_cursor_;
     ka.trace = lc_.trace ? d_trace_ : nullptr;
+    if (tile_[4] > 0) {
+        ka.tx0 = tile_[0]; ka.ty0 = tile_[1]; ka.tz0 = tile_[2]; ka.tg0 = tile_[3];
+        ka.tsx = tile_[4]; ka.tsy = tile_[5]; ka.tsz = tile_[6]; ka.tsg = tile_[7];
+        ka.tcells = ka.tsx * ka.tsy * ka.tsz;
+        ka.tile_n = ka.tcells * ka.tsg;
+    }
+    return out;
+}
 
+size_t Session::lds_bytes(uint32_t tile_n) const {
+    return (size_t)prob_.mediumnum * 16 + (size_t)tile_n * 4 + (size_t)prob_.ppath_columns() * block_ * 4;
+}
+
+// Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
+void Session::choose_tile(uint64_t first, uint64_t count) {
+    tile_chosen_ = true;
+    const Problem& p = prob_;
+    if (!p.issavevol || tally_mode_ == 0 || tile_budget_ < 64) return;
+    hipStream_t s = (hipStream_t)stream_;
+    uint64_t pilot = std::min<uint64_t>(count, 262144);
+    float* d_cnt = dalloc<float>(fluence_len_);
+    double* d_e = dalloc<double>(4);
+    unsigned long long* d_c = dalloc<unsigned long long>(2);
+    HIP_OK(hipMemsetAsync(d_cnt, 0, fluence_len_ * 4, s));
+    HIP_OK(hipMemsetAsync(d_e, 0, 32, s));
+    HIP_OK(hipMemsetAsync(d_c, 0, 16, s));
+    dev::KernelArgs ka;
+    build_args(first, pilot, &ka);
+    ka.fluence = d_cnt; ka.energy = d_e; ka.cursor = d_c; ka.trace = nullptr;
+    ka.detnum = 0; ka.ppathlen = 0; ka.count_mode = 1;
+    ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
+    const bool reflect = p.isreflect != 0;
+    pick(reflect, false, label_bytes_ == 1)(ka, blocks_, block_, (size_t)p.mediumnum * 16, s);
+    HIP_OK(hipGetLastError());
+    std::vector<float> cnt(fluence_len_);
+    HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
+    std::vector<int> box = choose_box(cnt, (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate, tile_budget_);
+    for (int i = 0; i < 8; i++) tile_[i] = box[i];
+}
+
+RunStats Session::run(uint64_t first, uint64_t count) {
+    HIP_OK(hipSetDevice(lc_.device));
+    hipStream_t s = (hipStream_t)stream_;
+    const Problem& p = prob_;
+    RunStats st;
+    st.photons = count;
+    last_count_ = count;
+    if (lc_.trace) {
+        if (trace_cap_ < count) {
+            if (d_trace_) HIP_OK(hipFree(d_trace_));
+            d_trace_ = dalloc<TraceRec>(count);
+            trace_cap_ = count;
+        }
+        HIP_OK(hipMemsetAsync(d_trace_, 0, count * sizeof(TraceRec), s));
+    }
+    if (count == 0) return st;
+    if (!tile_chosen_ && (tally_mode_ == 1 || (tally_mode_ < 0 && count >= 1000000))) choose_tile(first, count);
+    dev::KernelArgs ka;
+    build_args(first, count, &ka);
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
-    size_t lds = (size_t)p.mediumnum * 16 + (size_t)ka.ppathlen * block_ * 4;
+    size_t lds = lds_bytes(ka.tile_n);
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));

@@ -54,7 +54,7 @@ public:
     const Problem& problem() const { return prob_; }
     int block() const { return block_; }
     int blocks() const { return blocks_; }
-    int tally_mode() const { return tally_mode_; }
+    int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
     float* device_fluence() const { return d_fluence_; }
@@ -82,6 +82,18 @@ private:
     uint64_t trace_cap_ = 0;
     uint64_t last_count_ = 0;
     uint32_t detcap_ = 0;
+    // LDS tally tile {x0, y0, z0, g0, sx, sy, sz, sg}; sx = 0 means none
+    int tile_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool tile_chosen_ = false;
+    uint64_t tile_budget_ = 0;
+
+    void* build_args(uint64_t first, uint64_t count, void* out) const;
+    size_t lds_bytes(uint32_t tile_n) const;
+    void choose_tile(uint64_t first, uint64_t count);
+
+public:
+    // the chosen LDS tally tile, {x0, y0, z0, g0, sx, sy, sz, sg} (all zero when none)
+    const int* tile() const { return tile_; }
 };
 
 // Replica checks: evaluate the device libm replicas on n inputs (fn: 0 logf, 1 sinf, 2 cosf,
