@@ -43,6 +43,7 @@ typedef struct umcx_launch {
     int32_t blocks;      /* persistent workgroups */
     int32_t tally_mode;  /* -1 tuned, 0 global atomics, 1 LDS tile */
     int32_t trace;       /* 1: keep every photon's final state (small runs) */
+    int32_t vote;        /* scatter-round threshold in lanes; 0 scatters as soon as a path ends, -1 tuned */
 } umcx_launch;
 
 /* Constants of a prepared problem: MCX_param (src/umcx.cpp:95-99) plus launch facts. */

@@ -53,7 +53,7 @@ def test_per_photon_state_matches_oracle(name):
     n = 3000 if name not in ("c3", "skinvessel") else 1500
     say(f"trace parity {name} n={n}")
     o = O.run(prob, nphoton=n, traces=True, nthreads=8)["traces"]
-    with U.Simulation(g["cfg"], trace=True) as sim:
+    with U.Simulation(g["cfg"], trace=True, vote=32 if name in ("c2", "cube60b", "c3") else 0) as sim:
         sim.run(0, n)
         t = sim.trace(n)
     exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
@@ -180,8 +180,8 @@ def test_photon_ranges_compose():
 def test_launch_shape_does_not_change_results():
     g = O.golden("cube60b")
     outs = []
-    for block, blocks in ((64, 7), (256, 0), (128, 33)):
-        with U.Simulation(g["cfg"], block=block, blocks=blocks) as sim:
+    for block, blocks, vote in ((64, 7, 0), (256, 0, 0), (128, 33, 0), (256, 0, 24), (256, 0, 48), (64, 5, 64)):
+        with U.Simulation(g["cfg"], block=block, blocks=blocks, vote=vote) as sim:
             sim.run()
             outs.append((sorted_rows(sim.detp()[0]), sim.energies()))
     for rows, e in outs[1:]:

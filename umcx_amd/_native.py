@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libumcx.so")
 
 class UmcxLaunch(C.Structure):
     _fields_ = [("device", C.c_int32), ("block", C.c_int32), ("blocks", C.c_int32),
-                ("tally_mode", C.c_int32), ("trace", C.c_int32)]
+                ("tally_mode", C.c_int32), ("trace", C.c_int32), ("vote", C.c_int32)]
 
 
 class UmcxInfo(C.Structure):

@@ -46,6 +46,7 @@ umcx::LaunchConfig to_launch(const umcx_launch* l) {
         lc.blocks = l->blocks;
         lc.tally_mode = l->tally_mode;
         lc.trace = l->trace != 0;
+        lc.vote = l->vote;
     }
     return lc;
 }

@@ -72,7 +72,7 @@ static const LogfEntry kLogfTab[16] = {
     {0x1.886e6037841edp-1, 0x1.1058bc8a07ee1p-2}, {0x1.767dcf5534862p-1, 0x1.4043057b6ee09p-2},
 };
 
-UMCX_HD float logf_host(float x) {
+UMCX_HD float logf_tab(float x, const LogfEntry* tab) {
     const double ln2 = 0x1.62e42fefa39efp-1;
     const double a0 = -0x1.00ea348b88334p-2, a1 = 0x1.5575b0be00b6ap-2, a2 = -0x1.ffffef20a4123p-2;
     uint32_t ix = f2u(x);
@@ -81,7 +81,7 @@ UMCX_HD float logf_host(float x) {
     int idx = (int)((t >> 19) & 15u);
     int k = (int32_t)t >> 23;
     float zf = u2f(ix - (t & 0xff800000u));
-    const LogfEntry e = kLogfTab[idx];
+    const LogfEntry e = tab[idx];
     double r = dfma((double)zf, e.invc, -1.0);
     double y0 = dfma((double)k, ln2, e.logc);
     double r2 = r * r;
@@ -91,23 +91,24 @@ UMCX_HD float logf_host(float x) {
     return (float)y;
 }
 
+UMCX_HD float logf_host(float x) { return logf_tab(x, kLogfTab); }
+
 // ---- sinf/cosf: x = n pi/2 + r, |r| <= pi/4, then odd/even polynomials in double ----
-struct SinCosPoly { double c0, c1, c2, c3, c4; };
 UMCX_HD uint32_t abstop12(float x) { return (f2u(x) >> 20) & 0x7ffu; }
 
-UMCX_HD float sincos_poly(double x, double x2, bool cos_branch, bool neg_even) {
-    // s1..s3 odd coefficients; c0..c4 even coefficients (negated for quadrants 2,3)
+// odd polynomial S(r) ~ sin r; S(-r) == -S(r) exactly (every fused step is sign-symmetric)
+UMCX_HD float sin_poly(double x, double x2) {
     const double s1 = -0x1.555545995a603p-3, s2 = 0x1.1107605230bc4p-7, s3 = -0x1.994eb3774cf24p-13;
-    if (!cos_branch) {
-        double x3 = x * x2;
-        double p = dfma(x2, s3, s2);
-        double x7 = x3 * x2;
-        double s = dfma(x3, s1, x);
-        return (float)dfma(x7, p, s);
-    }
-    const double sg = neg_even ? -1.0 : 1.0;
-    const double c0 = sg * 0x1p0, c1 = sg * -0x1.ffffffd0c621cp-2, c2 = sg * 0x1.55553e1068f19p-5,
-                 c3 = sg * -0x1.6c087e89a359dp-10, c4 = sg * 0x1.99343027bf8c3p-16;
+    double x3 = x * x2;
+    double p = dfma(x2, s3, s2);
+    double x7 = x3 * x2;
+    double s = dfma(x3, s1, x);
+    return (float)dfma(x7, p, s);
+}
+// even polynomial C(r) ~ cos r; the host library's negated-coefficient table gives exactly -C(r)
+UMCX_HD float cos_poly(double x2) {
+    const double c0 = 0x1p0, c1 = -0x1.ffffffd0c621cp-2, c2 = 0x1.55553e1068f19p-5,
+                 c3 = -0x1.6c087e89a359dp-10, c4 = 0x1.99343027bf8c3p-16;
     double x4 = x2 * x2;
     double q2 = dfma(x2, c4, c3);
     double q1 = dfma(x2, c1, c0);
@@ -126,23 +127,24 @@ UMCX_HD double reduce_quadrant(double x, int* n) {
     return dfma(-(double)q, hpi, x);
 }
 
-// sinf(y) and cosf(y) of the same argument, each bitwise equal to the separate host calls.
+// sinf(y) and cosf(y) of one argument, each bitwise equal to the separate host calls. Both
+// polynomials are evaluated once and the quadrant picks them with signs:
+//   n%4 = 0: (S, C)   1: (C, -S)   2: (-S, -C)   3: (-C, S)
 UMCX_HD void sincosf_host(float y, float* s, float* c) {
-    double x = y;
-    if (abstop12(y) < abstop12(0x1.921FB6p-1f)) {          // |y| < pi/4
-        if (abstop12(y) < abstop12(0x1p-12f)) { *s = y; *c = 1.0f; return; }
-        double x2 = x * x;
-        *s = sincos_poly(x, x2, false, false);
-        *c = sincos_poly(x, x2, true, false);
-        return;
-    }
-    int n;
-    double r = reduce_quadrant(x, &n);
-    const double sgn = ((n & 3) == 1 || (n & 3) == 2) ? -1.0 : 1.0;   // {1,-1,-1,1}
-    double rs = r * sgn, r2 = r * r;
-    bool neg = (n & 2) != 0;
-    *s = sincos_poly(rs, r2, (n & 1) != 0, neg);
-    *c = sincos_poly(rs, r2, ((n ^ 1) & 1) != 0, neg);
+    const uint32_t top = abstop12(y);
+    double r = (double)y;
+    int n = 0;
+    if (top >= abstop12(0x1.921FB6p-1f)) r = reduce_quadrant(r, &n);   // |y| >= pi/4
+    const double r2 = r * r;
+    const float S = sin_poly(r, r2), C = cos_poly(r2);
+    const int q = n & 3;
+    float sv = (q & 1) ? C : S;
+    float cv = (q & 1) ? S : C;
+    sv = (q & 2) ? -sv : sv;
+    cv = (q == 1 || q == 2) ? -cv : cv;
+    const bool tiny = top < abstop12(0x1p-12f);
+    *s = tiny ? y : sv;
+    *c = tiny ? 1.0f : cv;
 }
 UMCX_HD float sinf_host(float y) { float s, c; sincosf_host(y, &s, &c); return s; }
 UMCX_HD float cosf_host(float y) { float s, c; sincosf_host(y, &s, &c); return c; }
@@ -166,28 +168,29 @@ UMCX_HD float sqrtf_cr(float x) {
 }
 UMCX_HD float acosf_host(float x) {
     const float pi = 3.1415925026e+00f, pio2_hi = 1.5707962513e+00f, pio2_lo = 7.5497894159e-08f;
-    int32_t hx = (int32_t)f2u(x);
-    int32_t ix = hx & 0x7fffffff;
-    if (ix == 0x3f800000) return hx > 0 ? 0.0f : pi + 2.0f * pio2_lo;
-    if (ix > 0x3f800000) return (x - x) / (x - x);
-    if (ix < 0x3f000000) {                                   // |x| < 0.5
+    const int32_t hx = (int32_t)f2u(x);
+    const int32_t ix = hx & 0x7fffffff;
+    if (ix >= 0x3f800000) {                                  // |x| >= 1
+        if (ix == 0x3f800000) return hx > 0 ? 0.0f : pi + 2.0f * pio2_lo;
+        return (x - x) / (x - x);
+    }
+    // the three ranges share one rational approximation R(z) = p(z)/q(z)
+    const bool small = ix < 0x3f000000;                      // |x| < 0.5: z = x^2
+    const bool neg = hx < 0;                                 // x <= -0.5: z = (1+x)/2, else (1-x)/2
+    const float z = small ? x * x : (neg ? (1.0f + x) * 0.5f : (1.0f - x) * 0.5f);
+    const float r = acos_rational(z);
+    if (small) {
         if (ix <= 0x23000000) return pio2_hi + pio2_lo;
-        float r = acos_rational(x * x);
         return pio2_hi - (x - (pio2_lo - x * r));
     }
-    if (hx < 0) {                                            // x <= -0.5
-        float z = (1.0f + x) * 0.5f;
-        float r = acos_rational(z);
-        float s = sqrtf_cr(z);
-        float w = r * s - pio2_lo;
+    const float s = sqrtf_cr(z);
+    if (neg) {
+        const float w = r * s - pio2_lo;
         return pi - 2.0f * (s + w);
     }
-    float z = (1.0f - x) * 0.5f;                             // x >= 0.5
-    float s = sqrtf_cr(z);
-    float df = u2f(f2u(s) & 0xfffff000u);
-    float c = (z - df * df) / (s + df);
-    float r = acos_rational(z);
-    float w = r * s + c;
+    const float df = u2f(f2u(s) & 0xfffff000u);
+    const float c = (z - df * df) / (s + df);
+    const float w = r * s + c;
     return 2.0f * (df + w);
 }
 
@@ -226,7 +229,7 @@ UMCX_HD uint64_t d2u64(double d) {
 
 // exp(x) for x in [-103.97, 88.72] (the photon loop only passes -mua*d <= 0); outside that
 // range it saturates to 0 / +inf like the host library.
-UMCX_HD float expf_host(float x) {
+UMCX_HD float expf_tab(float x, const uint64_t* tab) {
     const double inv_ln2_n = 0x1.71547652b82fep+0 * 32, shift = 0x1.8p+52;
     const double c0 = 0x1.c6af84b912394p-5 / 32 / 32 / 32, c1 = 0x1.ebfce50fac4f3p-3 / 32 / 32,
                  c2 = 0x1.62e42ff0c52d6p-1 / 32;
@@ -238,7 +241,7 @@ UMCX_HD float expf_host(float x) {
     uint64_t ki = d2u64(kd);
     kd -= shift;
     double r = dfma(inv_ln2_n, xd, -kd);
-    uint64_t t = kExp2Tab[ki % 32] + (ki << 47);
+    uint64_t t = tab[ki % 32] + (ki << 47);
     double s = u642d(t);
     double z = dfma(c0, r, c1);
     double r2 = r * r;
@@ -246,6 +249,8 @@ UMCX_HD float expf_host(float x) {
     y = dfma(z, r2, y);
     return (float)(y * s);
 }
+
+UMCX_HD float expf_host(float x) { return expf_tab(x, kExp2Tab); }
 
 // ---- nextafterf(x, y) by bit stepping (IEEE 754 semantics, denormals kept) ----
 UMCX_HD float nextafterf_host(float x, float y) {

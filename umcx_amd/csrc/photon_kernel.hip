@@ -62,6 +62,15 @@ struct KernelArgs {
     int32_t tx0, ty0, tz0, tg0;
     uint32_t tsx, tsy, tsz, tsg, tcells, tile_n;
     int32_t count_mode;      // 1: tally the number of deposits instead of their values
+    int32_t vote;            // 0: scatter right after the step that ends a free path; k > 0: a wave
+                             // scatters only once k of its lanes (or all of them) wait to scatter
+};
+
+// libm tables staged in LDS (per-lane gathers from constant memory would go through the vector
+// memory pipeline)
+struct Tabs {
+    const libm::LogfEntry* log;
+    const uint64_t* exp;
 };
 
 // ---------------- xorshift128+ (src/umcx.cpp:186-215) ----------------
@@ -80,8 +89,8 @@ __device__ __forceinline__ float rng_u01(Rng& r) {
     uint32_t lo = (uint32_t)(r.t1 + s0);
     return __builtin_bit_cast(float, 0x3F800000u | (lo >> 9)) - 1.0f;
 }
-__device__ __forceinline__ float rng_scatlen(Rng& r) {
-    return -libm::logf_host(rng_u01(r) + kEps);
+__device__ __forceinline__ float rng_scatlen(Rng& r, const Tabs& t) {
+    return -libm::logf_tab(rng_u01(r) + kEps, t.log);
 }
 
 // ---------------- photon state (src/umcx.cpp:216-220) ----------------
@@ -216,7 +225,7 @@ __device__ __forceinline__ bool skip_to_box(Photon& p, const KernelArgs& a) {
 }
 
 // step (:315-338); returns the voxel the photon is in afterwards (-1 outside)
-__device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a) {
+__device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a, const Tabs& t) {
     const float mua = prop.x, mus = prop.y, n = prop.w;
     float hx = fabsf(((float)(p.ix + (p.vx > 0.f)) - p.px) * p.rx);
     float hy = fabsf(((float)(p.iy + (p.vy > 0.f)) - p.py) * p.ry);
@@ -229,7 +238,7 @@ __device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a)
     p.px = p.px + d * p.vx;
     p.py = p.py + d * p.vy;
     p.pz = p.pz + d * p.vz;
-    p.pw = p.pw * libm::expf_host(-mua * d);
+    p.pw = p.pw * libm::expf_tab(-mua * d, t.exp);
     p.lx -= used;
     p.ly += d * n * kOneOverC0;
     p.lz += d;
@@ -245,9 +254,9 @@ __device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a)
 }
 
 // scatter (:364-380)
-__device__ __forceinline__ void scatter(Photon& p, Rng& r, float4 prop) {
+__device__ __forceinline__ void scatter(Photon& p, Rng& r, float4 prop, const Tabs& t) {
     const float g = prop.z;
-    p.lx = rng_scatlen(r);
+    p.lx = rng_scatlen(r, t);
     float sphi, cphi, sth, cth;
     libm::sincosf_host((2.f * kPi) * rng_u01(r), &sphi, &cphi);
     float th;
@@ -337,12 +346,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 template <bool REFLECT, bool SAVEDET, typename LabelT>
 __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
-    // LDS: media | tally tile | per-lane partial paths
+    // LDS: libm tables | media | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
-    float4* smedia = lds4;
-    float* stile = (float*)(lds4 + a.mediumnum);
+    libm::LogfEntry* slog = (libm::LogfEntry*)lds4;            // 16 x 16 B
+    uint64_t* sexp = (uint64_t*)(lds4 + 16);                     // 32 x 8 B
+    float4* smedia = lds4 + 32;
+    float* stile = (float*)(smedia + a.mediumnum);
     float* sppath = stile + a.tile_n;
     const int tid = threadIdx.x, bd = blockDim.x;
+    if (tid < 16) slog[tid] = libm::kLogfTab[tid];
+    if (tid < 32) sexp[tid] = libm::kExp2Tab[tid];
+    const Tabs tabs{slog, sexp};
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
     for (int i = tid; i < (int)a.tile_n; i += bd) stile[i] = 0.f;
     if (SAVEDET)
@@ -392,15 +406,22 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             }
             p.tvox = tile_local(a, p.ix, p.iy, p.iz);
             p.med = load_label<LabelT>(a.labels, p.lastvox);                                         // :269
-            p.lx = rng_scatlen(r);                                                                   // :270
+            p.lx = rng_scatlen(r, tabs);                                                             // :270
             alive = true;
         }
 
         bool ended = false;
+        // vote > 0: a scatter round runs only when enough lanes wait to scatter; in between, the
+        // waiting lanes idle and the others keep stepping
+        bool scatter_round = true;
+        if (a.vote > 0) {
+            const unsigned long long act = __ballot(1), sc = __ballot(!(p.lx > 0.f));
+            scatter_round = __popcll(sc) >= (unsigned)a.vote || sc == act;
+        }
         // ---- one step of sprint (:282-311) ----
-        if (p.lx > 0.f) {
+        if (p.lx > 0.f && (a.vote == 0 || !scatter_round)) {
             const float4 prop = smedia[p.med & kMed];
-            int nv = step(p, prop, a);
+            int nv = step(p, prop, a, tabs);
             if (nv != p.lastvox) {
                 const uint32_t m = p.med & kMed;
                 if (SAVEDET && a.ppathlen && m > 0) sppath[(m - 1) * bd + tid] += p.lz;
@@ -439,9 +460,10 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 }
             }
         }
-        // ---- scatter when the free path is used up (:272-274) ----
         if (!ended) {
-            while (p.lx <= 0.f) scatter(p, r, smedia[p.med & kMed]);
+            // a free path used up scatters (:272-274); a zero-length new path scatters again on
+            // the next pass, before any step, as the reference's loop does
+            if (scatter_round && !(p.lx > 0.f)) scatter(p, r, smedia[p.med & kMed], tabs);
             continue;
         }
 
@@ -743,6 +765,7 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     ka.energy = d_energy_;
     ka.cursor = d_cursor_;
     ka.trace = lc_.trace ? d_trace_ : nullptr;
+    ka.vote = lc_.vote < 0 ? 0 : std::min(lc_.vote, 64);
     if (tile_[4] > 0) {
         ka.tx0 = tile_[0]; ka.ty0 = tile_[1]; ka.tz0 = tile_[2]; ka.tg0 = tile_[3];
         ka.tsx = tile_[4]; ka.tsy = tile_[5]; ka.tsz = tile_[6]; ka.tsg = tile_[7];
@@ -753,7 +776,7 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
 }
 
 size_t Session::lds_bytes(uint32_t tile_n) const {
-    return (size_t)prob_.mediumnum * 16 + (size_t)tile_n * 4 + (size_t)prob_.ppath_columns() * block_ * 4;
+    return 32 * 16 + (size_t)prob_.mediumnum * 16 + (size_t)tile_n * 4 + (size_t)prob_.ppath_columns() * block_ * 4;
 }
 
 // Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
@@ -775,7 +798,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     ka.detnum = 0; ka.ppathlen = 0; ka.count_mode = 1;
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     const bool reflect = p.isreflect != 0;
-    pick(reflect, false, label_bytes_ == 1)(ka, blocks_, block_, (size_t)p.mediumnum * 16, s);
+    pick(reflect, false, label_bytes_ == 1)(ka, blocks_, block_, lds_bytes(0) - (size_t)p.ppath_columns() * block_ * 4, s);
     HIP_OK(hipGetLastError());
     std::vector<float> cnt(fluence_len_);
     HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));

@@ -18,6 +18,7 @@ struct LaunchConfig {
     int block = 0;           // threads per workgroup (0: tuned default)
     int blocks = 0;          // persistent workgroups (0: fill the GPU)
     int tally_mode = -1;     // -1 tuned, 0 global atomics, 1 LDS-cached tile
+    int vote = -1;           // scatter-round threshold in lanes (0: scatter right away, -1: tuned)
     bool trace = false;      // keep each photon's final state (parity tests; small runs only)
 };
 
