@@ -80,6 +80,28 @@ int umcx_session_detp(const umcx_session* s, float* rows, uint32_t* ids, float* 
 int umcx_session_energies(const umcx_session* s, double out[3]);
 /* Per-photon final state of the last run (64 bytes each), when launched with trace = 1. */
 int umcx_session_trace(const umcx_session* s, void* out, uint64_t n);
+/* ---- differentiation (no counterpart in the reference) ----------------------------------- */
+
+/* Per-voxel absorption and scattering coefficients in 1/mm (nx*ny*nz each, [x][y][z] order)
+ * overriding the labels' mua/mus; g and n stay per label. Device pointers when on_device = 1,
+ * host pointers otherwise; NULL restores the labels' values. With values equal to the labels'
+ * media the simulation is bitwise unchanged. */
+int umcx_session_set_voxel_props(umcx_session* s, const float* mua, const float* mus, int32_t on_device);
+/* Move the source (position and direction, grid units) for the following runs. */
+int umcx_session_set_source(umcx_session* s, const float pos[3], const float dir[3]);
+/* Device pointers of the detected-photon arrays (photon index, final weight, rows). */
+int umcx_session_detp_device(const umcx_session* s, uint32_t** ids, float** weights, float** rows);
+/* Replay: re-trace photons ids[0..n) (device arrays) with the session's current source and
+ * properties, adding for record k, in every voxel v it crosses,
+ *   grad_mua[slices[k]*slice_stride + v] += coef[k] * (-L_v)
+ *   grad_mus[slices[k]*slice_stride + v] += coef[k] * (n_v / mus_v - L_v)
+ * (L_v path length in mm, n_v scatterings, mus_v in 1/mm): with coef = w / I these are
+ * d log I / d mua_v and d log I / d mus_v of a detected intensity I = sum of final weights w
+ * (perturbation Monte Carlo). final_w (device, may be NULL) receives each replayed photon's
+ * final weight. All pointers are device pointers. */
+int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n,
+                        float* grad_mua, float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms);
+
 /* The LDS tally tile in use, {x0, y0, z0, gate0, nx, ny, nz, ngates} (zeros when none).
  * No counterpart in the reference. */
 int umcx_session_tally_box(const umcx_session* s, int32_t box[8]);

@@ -47,6 +47,12 @@ SIGNATURES = {
     "umcx_session_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "umcx_session_normalizer": (C.c_float, [C.c_void_p, C.c_double]),
     "umcx_session_tally_box": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "umcx_session_set_voxel_props": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]),
+    "umcx_session_set_source": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "umcx_session_detp_device": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p)]),
+    "umcx_session_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                      C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_double)]),
     "umcx_session_free": (None, [C.c_void_p]),
     "umcx_domain_from_json": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     "umcx_config_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),

@@ -238,6 +238,32 @@ int umcx_session_trace(const umcx_session* s, void* out, uint64_t n) {
     return guarded([&]() -> int { s->s->fetch_trace((umcx::TraceRec*)out, n); return 0; });
 }
 
+int umcx_session_set_voxel_props(umcx_session* s, const float* mua, const float* mus, int32_t on_device) {
+    return guarded([&]() -> int { s->s->set_voxel_props(mua, mus, on_device != 0); return 0; });
+}
+
+int umcx_session_set_source(umcx_session* s, const float pos[3], const float dir[3]) {
+    return guarded([&]() -> int { s->s->set_source(pos, dir); return 0; });
+}
+
+int umcx_session_detp_device(const umcx_session* s, uint32_t** ids, float** weights, float** rows) {
+    return guarded([&]() -> int {
+        if (ids) *ids = s->s->device_detid();
+        if (weights) *weights = s->s->device_detw();
+        if (rows) *rows = s->s->device_detp();
+        return 0;
+    });
+}
+
+int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n,
+                        float* grad_mua, float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms) {
+    return guarded([&]() -> int {
+        umcx::RunStats st = s->s->replay(ids, slices, coef, n, grad_mua, grad_mus, slice_stride, final_w);
+        if (kernel_ms) *kernel_ms = st.kernel_ms;
+        return 0;
+    });
+}
+
 int umcx_session_tally_box(const umcx_session* s, int32_t box[8]) {
     return guarded([&]() -> int {
         for (int i = 0; i < 8; i++) box[i] = s->s->tile()[i];

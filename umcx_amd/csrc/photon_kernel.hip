@@ -64,6 +64,17 @@ struct KernelArgs {
     int32_t count_mode;      // 1: tally the number of deposits instead of their values
     int32_t vote;            // 0: scatter right after the step that ends a free path; k > 0: a wave
                              // scatters only once k of its lanes (or all of them) wait to scatter
+    // per-voxel {mua, mus} in per-voxel-length units overriding the labels' (nullptr: labels only)
+    const float2* vprops;
+    // replay (Jacobian): photon ids, output slice and coefficient per record; gradients out
+    const uint32_t* rids;
+    const int32_t* rslice;
+    const float* rcoef;
+    float* gmua;
+    float* gmus;
+    uint64_t gstride;
+    float unit;              // mm per voxel
+    float* rw;               // final weight of each replayed photon (may be null)
 };
 
 // libm tables staged in LDS (per-lane gathers from constant memory would go through the vector
@@ -103,7 +114,21 @@ struct Photon {
     int lastvox;
     uint32_t med;
     int tvox;                  // offset of lastvox inside the LDS tally tile, -1 outside it
+    float cmua, cmus;          // absorption and scattering coefficients of lastvox
 };
+
+// coefficients of the photon's current voxel: the label's medium, or the per-voxel override
+__device__ __forceinline__ void load_props(Photon& p, const KernelArgs& a, const float4* smedia) {
+    if (a.vprops) {
+        const float2 v = a.vprops[p.lastvox];
+        p.cmua = v.x;
+        p.cmus = v.y;
+    } else {
+        const float4 m = smedia[p.med & kMed];
+        p.cmua = m.x;
+        p.cmus = m.y;
+    }
+}
 
 template <typename LabelT>
 __device__ __forceinline__ uint32_t load_label(const void* labels, int idx);
@@ -225,8 +250,8 @@ __device__ __forceinline__ bool skip_to_box(Photon& p, const KernelArgs& a) {
 }
 
 // step (:315-338); returns the voxel the photon is in afterwards (-1 outside)
-__device__ __forceinline__ int step(Photon& p, float4 prop, const KernelArgs& a, const Tabs& t) {
-    const float mua = prop.x, mus = prop.y, n = prop.w;
+__device__ __forceinline__ int step(Photon& p, float n, const KernelArgs& a, const Tabs& t) {
+    const float mua = p.cmua, mus = p.cmus;
     float hx = fabsf(((float)(p.ix + (p.vx > 0.f)) - p.px) * p.rx);
     float hy = fabsf(((float)(p.iy + (p.vy > 0.f)) - p.py) * p.ry);
     float hz = fabsf(((float)(p.iz + (p.vz > 0.f)) - p.pz) * p.rz);
@@ -344,7 +369,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-template <bool REFLECT, bool SAVEDET, typename LabelT>
+// REPLAY: re-trace the photons listed in a.rids (photon i is a pure function of i) and add
+// each voxel segment's perturbation-MC terms to the gradient slices instead of tallying.
+template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY>
 __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     // LDS: libm tables | media | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
@@ -369,7 +396,9 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     uint64_t pid = 0;
     float w0 = 0.f;
     double e_esc = 0.0, e_tot = 0.0, e_dep = 0.0, ph_dep = 0.0;
-    const bool savevol = a.issavevol != 0;
+    float rcoef = 0.f, seg_ns = 0.f;
+    uint64_t rbase = 0, rk = 0;
+    const bool savevol = !REPLAY && a.issavevol != 0;
     const bool energy_out = a.outputtype == kEnergy;
 
     while (true) {
@@ -382,7 +411,13 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             base = __shfl(base, leader);
             const unsigned long long k = base + (unsigned long long)__popcll(need & lanemask_lt());
             if (k >= a.count) break;
-            pid = a.first + k;
+            pid = REPLAY ? (uint64_t)a.rids[k] : a.first + k;
+            if (REPLAY) {
+                rk = k;
+                rcoef = a.rcoef[k];
+                rbase = (uint64_t)a.rslice[k] * a.gstride;
+                seg_ns = 0.f;
+            }
             const uint32_t i32 = (uint32_t)pid;
             rng_seed(r, a.seeds[0] ^ i32, a.seeds[1] | i32, a.seeds[2] ^ i32, a.seeds[3] | i32);   // :761
             w0 = launch(p, r, a);
@@ -406,6 +441,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             }
             p.tvox = tile_local(a, p.ix, p.iy, p.iz);
             p.med = load_label<LabelT>(a.labels, p.lastvox);                                         // :269
+            load_props(p, a, smedia);
             p.lx = rng_scatlen(r, tabs);                                                             // :270
             alive = true;
         }
@@ -421,14 +457,23 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         // ---- one step of sprint (:282-311) ----
         if (p.lx > 0.f && (a.vote == 0 || !scatter_round)) {
             const float4 prop = smedia[p.med & kMed];
-            int nv = step(p, prop, a, tabs);
+            int nv = step(p, prop.w, a, tabs);
             if (nv != p.lastvox) {
                 const uint32_t m = p.med & kMed;
+                if (REPLAY) {
+                    // d w / d mua_v = -w L_v;  d log p / d mus_v = n_v / mus_v - L_v   (mm units)
+                    const float L = p.lz * a.unit;
+                    const float dmus = (seg_ns > 0.f ? seg_ns * a.unit / p.cmus : 0.f) - L;
+                    unsafeAtomicAdd(a.gmua + rbase + (uint64_t)p.lastvox, -rcoef * L);
+                    unsafeAtomicAdd(a.gmus + rbase + (uint64_t)p.lastvox, rcoef * dmus);
+                    p.lz = 0.f;
+                    seg_ns = 0.f;
+                }
                 if (SAVEDET && a.ppathlen && m > 0) sppath[(m - 1) * bd + tid] += p.lz;
                 if (savevol) {
                     int tshift = (int)fminf((float)(a.maxgate - 1), (float)(int)floorf((p.ly - a.tstart) * a.rtstep));
                     float dw = p.lw - p.pw;
-                    float val = energy_out ? dw : ((prop.x < kEps) ? (p.lw * p.lz) : dw / prop.x);
+                    float val = energy_out ? dw : ((p.cmua < kEps) ? (p.lw * p.lz) : dw / p.cmua);
                     ph_dep += (double)dw;
                     if (a.count_mode) val = 1.f;
                     if (tshift >= 0) {
@@ -453,9 +498,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                         ended = true;
                     }
                     if (!ended) {
-                        if (nv != p.lastvox) p.tvox = tile_local(a, p.ix, p.iy, p.iz);
+                        const bool moved = nv != p.lastvox;
+                        if (moved) p.tvox = tile_local(a, p.ix, p.iy, p.iz);
                         p.lastvox = nv;
                         p.med = nm;
+                        if (moved) load_props(p, a, smedia);
                     }
                 }
             }
@@ -463,10 +510,18 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         if (!ended) {
             // a free path used up scatters (:272-274); a zero-length new path scatters again on
             // the next pass, before any step, as the reference's loop does
-            if (scatter_round && !(p.lx > 0.f)) scatter(p, r, smedia[p.med & kMed], tabs);
+            if (scatter_round && !(p.lx > 0.f)) {
+                scatter(p, r, smedia[p.med & kMed], tabs);
+                if (REPLAY) seg_ns += 1.f;
+            }
             continue;
         }
 
+        if (REPLAY) {
+            if (a.rw) a.rw[rk] = p.pw;
+            alive = false;
+            continue;
+        }
         // ---- photon ends: detection (:276-278) and energy tallies (:765-770) ----
         if (SAVEDET && (p.med & kDet) && p.ly <= a.tend) {
             for (int d = 0; d < a.detnum; d++) {
@@ -534,6 +589,12 @@ __global__ void zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// per-voxel {mua, mus} in 1/mm -> per-voxel-length units, as the label media are scaled (:804)
+__global__ void vprops_kernel(const float* mua, const float* mus, float unit, float2* out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(mua[i] * unit, mus[i] * unit);
+}
+
 __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -554,12 +615,17 @@ __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n
 
 namespace {
 
-template <bool R, bool D, typename L>
+using LaunchFn = void (*)(const dev::KernelArgs&, int, int, size_t, hipStream_t);
+
+template <bool R, bool D, typename L, bool P = false>
 void launch_one(const dev::KernelArgs& ka, int blocks, int block, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((dev::photon_kernel<R, D, L>), dim3(blocks), dim3(block), lds, s, ka);
+    hipLaunchKernelGGL((dev::photon_kernel<R, D, L, P>), dim3(blocks), dim3(block), lds, s, ka);
 }
 
-using LaunchFn = void (*)(const dev::KernelArgs&, int, int, size_t, hipStream_t);
+LaunchFn pick_replay(bool reflect, bool u8) {
+    if (u8) return reflect ? launch_one<true, false, uint8_t, true> : launch_one<false, false, uint8_t, true>;
+    return reflect ? launch_one<true, false, uint32_t, true> : launch_one<false, false, uint32_t, true>;
+}
 
 LaunchFn pick(bool reflect, bool savedet, bool u8) {
     if (u8) {
@@ -572,11 +638,11 @@ LaunchFn pick(bool reflect, bool savedet, bool u8) {
 
 const void* kernel_ptr(bool reflect, bool savedet, bool u8) {
     if (u8) {
-        if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint8_t> : (const void*)dev::photon_kernel<true, false, uint8_t>;
-        return savedet ? (const void*)dev::photon_kernel<false, true, uint8_t> : (const void*)dev::photon_kernel<false, false, uint8_t>;
+        if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint8_t, false> : (const void*)dev::photon_kernel<true, false, uint8_t, false>;
+        return savedet ? (const void*)dev::photon_kernel<false, true, uint8_t, false> : (const void*)dev::photon_kernel<false, false, uint8_t, false>;
     }
-    if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint32_t> : (const void*)dev::photon_kernel<true, false, uint32_t>;
-    return savedet ? (const void*)dev::photon_kernel<false, true, uint32_t> : (const void*)dev::photon_kernel<false, false, uint32_t>;
+    if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint32_t, false> : (const void*)dev::photon_kernel<true, false, uint32_t, false>;
+    return savedet ? (const void*)dev::photon_kernel<false, true, uint32_t, false> : (const void*)dev::photon_kernel<false, false, uint32_t, false>;
 }
 
 template <typename T>
@@ -673,7 +739,7 @@ Session::~Session() {
     (void)hipSetDevice(lc_.device);
     for (void* ptr : {(void*)d_labels_, (void*)d_media_, (void*)d_detpos_, (void*)d_fluence_, (void*)d_detp_,
                       (void*)d_detid_, (void*)d_detw_, (void*)d_counters_, (void*)d_cursor_, (void*)d_energy_,
-                      (void*)d_trace_})
+                      (void*)d_trace_, d_vprops_})
         if (ptr) (void)hipFree(ptr);
     if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
     if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
@@ -766,6 +832,8 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     ka.cursor = d_cursor_;
     ka.trace = lc_.trace ? d_trace_ : nullptr;
     ka.vote = lc_.vote < 0 ? 0 : std::min(lc_.vote, 64);
+    ka.vprops = (const float2*)d_vprops_;
+    ka.unit = p.unitinmm;
     if (tile_[4] > 0) {
         ka.tx0 = tile_[0]; ka.ty0 = tile_[1]; ka.tz0 = tile_[2]; ka.tg0 = tile_[3];
         ka.tsx = tile_[4]; ka.tsy = tile_[5]; ka.tsz = tile_[6]; ka.tsg = tile_[7];
@@ -833,6 +901,71 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
     pick(reflect, savedet, label_bytes_ == 1)(ka, blocks_, block_, lds, s);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
+    HIP_OK(hipStreamSynchronize(s));
+    float ms = 0.f;
+    HIP_OK(hipEventElapsedTime(&ms, (hipEvent_t)ev0_, (hipEvent_t)ev1_));
+    st.kernel_ms = ms;
+    return st;
+}
+
+void Session::set_voxel_props(const float* mua, const float* mus, bool on_device) {
+    HIP_OK(hipSetDevice(lc_.device));
+    hipStream_t s = (hipStream_t)stream_;
+    const uint64_t n = (uint64_t)prob_.nx * prob_.ny * prob_.nz;
+    if (!mua || !mus) {
+        if (d_vprops_) HIP_OK(hipFree(d_vprops_));
+        d_vprops_ = nullptr;
+        return;
+    }
+    if (!d_vprops_) d_vprops_ = dalloc<float2>(n);
+    const float *dm = mua, *ds = mus;
+    float* staging = nullptr;
+    if (!on_device) {
+        staging = dalloc<float>(2 * n);
+        HIP_OK(hipMemcpyAsync(staging, mua, n * 4, hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemcpyAsync(staging + n, mus, n * 4, hipMemcpyHostToDevice, s));
+        dm = staging;
+        ds = staging + n;
+    }
+    hipLaunchKernelGGL(dev::vprops_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dm, ds, prob_.unitinmm,
+                       (float2*)d_vprops_, n);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    if (staging) HIP_OK(hipFree(staging));
+    tile_chosen_ = false;
+    for (int& v : tile_) v = 0;
+}
+
+void Session::set_source(const float pos[3], const float dir[3]) {
+    for (int i = 0; i < 3; i++) {
+        prob_.srcpos[i] = pos[i];
+        prob_.srcdir[i] = dir[i];
+    }
+    tile_chosen_ = false;
+    for (int& v : tile_) v = 0;
+}
+
+RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n, float* gmua,
+                         float* gmus, uint64_t stride, float* final_w) {
+    HIP_OK(hipSetDevice(lc_.device));
+    hipStream_t s = (hipStream_t)stream_;
+    RunStats st;
+    st.photons = n;
+    if (n == 0) return st;
+    dev::KernelArgs ka;
+    build_args(0, n, &ka);
+    ka.trace = nullptr;
+    ka.detnum = 0; ka.ppathlen = 0; ka.issavevol = 0;
+    ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
+    ka.rids = ids; ka.rslice = slices; ka.rcoef = coef;
+    ka.gmua = gmua; ka.gmus = gmus; ka.gstride = stride; ka.rw = final_w;
+    ka.vote = 0;
+    hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
+    pick_replay(prob_.isreflect != 0, label_bytes_ == 1)(ka, blocks_, block_, lds_bytes(0) - (size_t)prob_.ppath_columns() * block_ * 4, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));

@@ -50,6 +50,18 @@ public:
     uint32_t saved() const;                              // rows kept (<= maxdetphotons)
     void fetch_detp(float* rows, uint32_t* ids, float* weights) const;
     void energies(double out[3]) const;                  // escaped, launched, deposited (fp64)
+    // per-voxel mua/mus in 1/mm overriding the labels' (null: back to labels); device or host
+    void set_voxel_props(const float* mua, const float* mus, bool on_device);
+    // move the source (position, direction) for the next runs
+    void set_source(const float pos[3], const float dir[3]);
+    // re-trace photons ids[0..n) (device arrays) adding coef[k] x perturbation-MC derivatives of
+    // the photon's weight (mua) and path density (mus) into slice slices[k] of gmua / gmus
+    // (device, slice stride `stride` floats, voxels in label order); final weights to final_w
+    RunStats replay(const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n, float* gmua,
+                    float* gmus, uint64_t stride, float* final_w);
+    uint32_t* device_detid() const { return d_detid_; }
+    float* device_detw() const { return d_detw_; }
+    float* device_detp() const { return d_detp_; }
     void fetch_trace(TraceRec* out, uint64_t n) const;   // of the last run
     uint64_t fluence_len() const { return fluence_len_; }
     const Problem& problem() const { return prob_; }
@@ -80,6 +92,7 @@ private:
     unsigned long long* d_cursor_ = nullptr;
     double* d_energy_ = nullptr;
     TraceRec* d_trace_ = nullptr;
+    void* d_vprops_ = nullptr;
     uint64_t trace_cap_ = 0;
     uint64_t last_count_ = 0;
     uint32_t detcap_ = 0;
