@@ -109,6 +109,20 @@ int umcx_session_tally_box(const umcx_session* s, int32_t box[8]);
 float umcx_session_normalizer(const umcx_session* s, double energytotal);
 void umcx_session_free(umcx_session* s);
 
+/* ---- batched simulations (no counterpart in the reference) ------------------------------- */
+
+/* Many independent simulations (each a complete umcx JSON config; fluence output only) run in
+ * one persistent launch replayed from a hipGraph. Per-job results equal separate runs up to float
+ * summation order. */
+typedef struct umcx_batch umcx_batch;
+int umcx_batch_create(const char* const* jsons, int32_t njobs, const umcx_launch* launch, umcx_batch** out);
+/* Zero the tallies and run every job's photons; kernel_ms (may be NULL) gets the graph's time. */
+int umcx_batch_run(umcx_batch* b, double* kernel_ms);
+int umcx_batch_info(const umcx_batch* b, int32_t job, umcx_info* info);
+int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n);
+int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]);
+void umcx_batch_free(umcx_batch* b);
+
 /* ---- host-only helpers (no GPU needed) ---------------------------------------------------- */
 
 /* Label volume with detector bits, as MCX_userio::initdomain + maskdetectors build it

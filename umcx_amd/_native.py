@@ -54,6 +54,12 @@ SIGNATURES = {
     "umcx_session_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                       C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_double)]),
     "umcx_session_free": (None, [C.c_void_p]),
+    "umcx_batch_create": (C.c_int, [C.POINTER(C.c_char_p), C.c_int32, C.POINTER(UmcxLaunch), C.POINTER(C.c_void_p)]),
+    "umcx_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "umcx_batch_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(UmcxInfo)]),
+    "umcx_batch_fluence": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]),
+    "umcx_batch_energies": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "umcx_batch_free": (None, [C.c_void_p]),
     "umcx_domain_from_json": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     "umcx_config_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),
     "umcx_seed_words": (C.c_int, [C.c_int32, C.POINTER(C.c_uint32)]),

@@ -11,6 +11,7 @@
 #include <memory>
 #include <string>
 
+#include "batch.h"
 #include "bjdata.h"
 #include "config.h"
 #include "glibc_libm.h"
@@ -18,6 +19,10 @@
 
 struct umcx_session {
     std::unique_ptr<umcx::Session> s;
+};
+
+struct umcx_batch {
+    std::unique_ptr<umcx::BatchSession> b;
 };
 
 namespace {
@@ -276,6 +281,68 @@ float umcx_session_normalizer(const umcx_session* s, double energytotal) {
 }
 
 void umcx_session_free(umcx_session* s) { delete s; }
+
+static void fill_info(const umcx::Problem& p, umcx_info* info) {
+    memset(info, 0, sizeof *info);
+    info->dims[0] = p.nx; info->dims[1] = p.ny; info->dims[2] = p.nz; info->dims[3] = (uint32_t)p.maxgate;
+    info->nphoton = p.nphoton;
+    info->mediumnum = p.mediumnum; info->detnum = p.detnum;
+    info->detp_columns = p.detp_columns(); info->ppath_columns = p.ppath_columns();
+    info->outputtype = p.outputtype; info->isnormalized = p.isnormalized; info->issavevol = p.issavevol;
+    info->issavedet = p.issavedet; info->isreflect = p.isreflect; info->srctype = p.srctype;
+    info->tstart = p.tstart; info->tend = p.tend; info->rtstep = p.rtstep; info->unitinmm = p.unitinmm;
+    for (int i = 0; i < 4; i++) info->seeds[i] = p.seeds[i];
+    info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
+}
+
+int umcx_batch_create(const char* const* jsons, int32_t njobs, const umcx_launch* launch, umcx_batch** out) {
+    return guarded([&]() -> int {
+        if (!jsons || njobs <= 0 || !out) throw std::runtime_error("umcx_batch_create: need at least one job");
+        std::vector<umcx::Problem> probs;
+        probs.reserve((size_t)njobs);
+        for (int32_t j = 0; j < njobs; j++) probs.push_back(umcx::make_problem(umcx::Json::parse(jsons[j])));
+        auto* b = new umcx_batch;
+        try {
+            b->b.reset(new umcx::BatchSession(probs, to_launch(launch)));
+        } catch (...) {
+            delete b;
+            throw;
+        }
+        *out = b;
+        return 0;
+    });
+}
+
+int umcx_batch_run(umcx_batch* b, double* kernel_ms) {
+    return guarded([&]() -> int {
+        umcx::RunStats st = b->b->run();
+        if (kernel_ms) *kernel_ms = st.kernel_ms;
+        return 0;
+    });
+}
+
+int umcx_batch_info(const umcx_batch* b, int32_t job, umcx_info* info) {
+    return guarded([&]() -> int {
+        fill_info(b->b->problem(job), info);
+        info->block = b->b->block();
+        info->blocks = b->b->blocks();
+        return 0;
+    });
+}
+
+int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n) {
+    return guarded([&]() -> int {
+        if (n < b->b->fluence_len(job)) throw std::runtime_error("umcx_batch_fluence: buffer too small");
+        b->b->fetch_fluence(job, out);
+        return 0;
+    });
+}
+
+int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]) {
+    return guarded([&]() -> int { b->b->energies(job, out); return 0; });
+}
+
+void umcx_batch_free(umcx_batch* b) { delete b; }
 
 int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels) {
     return guarded([&]() -> int {
