@@ -121,6 +121,18 @@ int umcx_batch_run(umcx_batch* b, double* kernel_ms);
 int umcx_batch_info(const umcx_batch* b, int32_t job, umcx_info* info);
 int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n);
 int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]);
+/* One launch shape the autotuner timed at creation: workgroup size, persistent workgroups, LDS
+ * tally cells per workgroup (0: global atomics only), the trial's kernel time over a 1/32 sample
+ * of every job, the share of pilot deposits inside the jobs' tiles, and whether it was chosen. */
+typedef struct umcx_batch_trial {
+    int32_t block, blocks, tile_cells, chosen;
+    double ms, tile_share;
+} umcx_batch_trial;
+/* Writes at most cap trials to out (may be NULL) and their number to *n. */
+int umcx_batch_trials(const umcx_batch* b, umcx_batch_trial* out, int32_t cap, int32_t* n);
+/* LDS tally box of one job, {x0, y0, z0, gate0, nx, ny, nz, ngates} (zeros when none), and its
+ * pilot deposits per photon. */
+int umcx_batch_tally_box(const umcx_batch* b, int32_t job, int32_t box[8], double* deposits_per_photon);
 void umcx_batch_free(umcx_batch* b);
 
 /* ---- host-only helpers (no GPU needed) ---------------------------------------------------- */

@@ -27,6 +27,11 @@ class UmcxInfo(C.Structure):
                 ("tally_mode", C.c_int32), ("label_bytes", C.c_int32)]
 
 
+class UmcxBatchTrial(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks", C.c_int32), ("tile_cells", C.c_int32), ("chosen", C.c_int32),
+                ("ms", C.c_double), ("tile_share", C.c_double)]
+
+
 # every exported symbol and its signature: (restype, argtypes)
 SIGNATURES = {
     "MCX_run_cmd": (C.c_int, [C.POINTER(C.c_char_p), C.c_int]),
@@ -59,6 +64,8 @@ SIGNATURES = {
     "umcx_batch_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(UmcxInfo)]),
     "umcx_batch_fluence": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]),
     "umcx_batch_energies": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "umcx_batch_trials": (C.c_int, [C.c_void_p, C.POINTER(UmcxBatchTrial), C.c_int32, C.POINTER(C.c_int32)]),
+    "umcx_batch_tally_box": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "umcx_batch_free": (None, [C.c_void_p]),
     "umcx_domain_from_json": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     "umcx_config_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),

@@ -17,7 +17,7 @@ import zlib
 
 import numpy as np
 
-from ._native import UmcxInfo, UmcxLaunch, check, lib
+from ._native import UmcxBatchTrial, UmcxInfo, UmcxLaunch, check, lib
 from .api import Simulation
 
 
@@ -63,10 +63,10 @@ def roi_sum(fluence: np.ndarray, cfg: dict, half: int = 2) -> float:
 class BatchSimulation:
     """All jobs prepared on the GPU; ``run()`` replays the captured graph."""
 
-    def __init__(self, cfgs: list, device: int = 0, block: int = 0, blocks: int = 0):
+    def __init__(self, cfgs: list, device: int = 0, block: int = 0, blocks: int = 0, tally_mode: int = -1):
         texts = [c if isinstance(c, str) else json.dumps(c) for c in cfgs]
         arr = (C.c_char_p * len(texts))(*[t.encode() for t in texts])
-        launch = UmcxLaunch(device, block, blocks, 0, 0, 0)
+        launch = UmcxLaunch(device, block, blocks, tally_mode, 0, 0)
         h = C.c_void_p()
         check(lib().umcx_batch_create(arr, len(texts), C.byref(launch), C.byref(h)))
         self._h = h
@@ -77,6 +77,21 @@ class BatchSimulation:
             check(lib().umcx_batch_info(h, j, C.byref(info)))
             self.infos.append({f: (list(getattr(info, f)) if f in ("dims", "seeds") else getattr(info, f))
                                for f, _ in UmcxInfo._fields_})
+
+    def trials(self) -> list[dict]:
+        """The autotuner's timed launch shapes (see umcx_batch_trials)."""
+        n = C.c_int32(0)
+        check(lib().umcx_batch_trials(self._h, None, 0, C.byref(n)))
+        arr = (UmcxBatchTrial * max(n.value, 1))()
+        check(lib().umcx_batch_trials(self._h, arr, n.value, C.byref(n)))
+        return [{f: getattr(arr[i], f) for f, _ in UmcxBatchTrial._fields_} for i in range(n.value)]
+
+    def tally_box(self, j: int):
+        """({x0, y0, z0, gate0, nx, ny, nz, ngates}, pilot deposits per photon) of job j."""
+        box = (C.c_int32 * 8)()
+        dep = C.c_double(0.0)
+        check(lib().umcx_batch_tally_box(self._h, j, box, C.byref(dep)))
+        return list(box), dep.value
 
     def run(self) -> float:
         ms = C.c_double(0.0)

@@ -326,6 +326,7 @@ int umcx_batch_info(const umcx_batch* b, int32_t job, umcx_info* info) {
         fill_info(b->b->problem(job), info);
         info->block = b->b->block();
         info->blocks = b->b->blocks();
+        info->tally_mode = b->b->tile_cells() > 0 ? 1 : 0;
         return 0;
     });
 }
@@ -340,6 +341,32 @@ int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n)
 
 int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]) {
     return guarded([&]() -> int { b->b->energies(job, out); return 0; });
+}
+
+int umcx_batch_trials(const umcx_batch* b, umcx_batch_trial* out, int32_t cap, int32_t* n) {
+    return guarded([&]() -> int {
+        const auto& t = b->b->trials();
+        if (n) *n = (int32_t)t.size();
+        for (int32_t i = 0; out && i < cap && i < (int32_t)t.size(); i++) {
+            out[i].block = t[i].block;
+            out[i].blocks = t[i].blocks;
+            out[i].tile_cells = t[i].tile_cells;
+            out[i].chosen = t[i].block == b->b->block() && t[i].blocks == b->b->blocks() &&
+                            t[i].tile_cells == b->b->tile_cells();
+            out[i].ms = t[i].ms;
+            out[i].tile_share = t[i].tile_share;
+        }
+        return 0;
+    });
+}
+
+int umcx_batch_tally_box(const umcx_batch* b, int32_t job, int32_t box[8], double* deposits_per_photon) {
+    return guarded([&]() -> int {
+        const umcx::TallyBox& bx = b->b->box(job);
+        for (int i = 0; i < 8; i++) box[i] = bx[i];
+        if (deposits_per_photon) *deposits_per_photon = b->b->cost(job);
+        return 0;
+    });
 }
 
 void umcx_batch_free(umcx_batch* b) { delete b; }

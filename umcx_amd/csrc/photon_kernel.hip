@@ -22,6 +22,7 @@
 #include "glibc_libm.h"
 #include "photon_device.h"
 #include "runtime.h"
+#include "tally_box.h"
 
 #define HIP_OK(x)                                                                                   \
     do {                                                                                            \
@@ -479,58 +480,6 @@ void Session::reset() {
     HIP_OK(hipStreamSynchronize(s));
 }
 
-namespace {
-
-// Greedy growth of an axis-aligned (gate, x, y, z) box around the densest cell of a count grid,
-// adding at each step the face slab with the most deposits per added cell, while the box holds at
-// most `budget` cells. Returns {x0, y0, z0, g0, sx, sy, sz, sg}.
-std::vector<int> choose_box(const std::vector<float>& cnt, int nx, int ny, int nz, int nt, uint64_t budget) {
-    const uint64_t nyz = (uint64_t)ny * nz, nxyz = nyz * nx;
-    auto at = [&](int g, int x, int y, int z) -> double { return cnt[(uint64_t)g * nxyz + (uint64_t)x * nyz + (uint64_t)y * nz + z]; };
-    uint64_t best = 0;
-    for (uint64_t i = 1; i < cnt.size(); i++) if (cnt[i] > cnt[best]) best = i;
-    int lo[4], hi[4];     // g, x, y, z half-open
-    lo[0] = (int)(best / nxyz); uint64_t r = best % nxyz;
-    lo[1] = (int)(r / nyz); r %= nyz;
-    lo[2] = (int)(r / nz); lo[3] = (int)(r % nz);
-    for (int d = 0; d < 4; d++) hi[d] = lo[d] + 1;
-    const int ext[4] = {nt, nx, ny, nz};
-    auto cells = [&](const int* l, const int* h) { uint64_t c = 1; for (int d = 0; d < 4; d++) c *= (uint64_t)(h[d] - l[d]); return c; };
-    auto slab = [&](int d, int at_idx) {
-        int l[4], h[4];
-        for (int k = 0; k < 4; k++) { l[k] = lo[k]; h[k] = hi[k]; }
-        l[d] = at_idx; h[d] = at_idx + 1;
-        double sum = 0;
-        for (int g = l[0]; g < h[0]; g++)
-            for (int x = l[1]; x < h[1]; x++)
-                for (int y = l[2]; y < h[2]; y++)
-                    for (int z = l[3]; z < h[3]; z++) sum += at(g, x, y, z);
-        return sum;
-    };
-    while (true) {
-        int bd = -1, bside = 0;
-        double bscore = 0;
-        for (int d = 0; d < 4; d++)
-            for (int side = 0; side < 2; side++) {
-                int idx = side ? hi[d] : lo[d] - 1;
-                if (idx < 0 || idx >= ext[d]) continue;
-                int l[4], h[4];
-                for (int k = 0; k < 4; k++) { l[k] = lo[k]; h[k] = hi[k]; }
-                if (side) h[d]++; else l[d]--;
-                uint64_t nc = cells(l, h);
-                if (nc > budget) continue;
-                double add = slab(d, idx);
-                double score = add / (double)(nc - cells(lo, hi));
-                if (add > 0 && score > bscore) { bscore = score; bd = d; bside = side; }
-            }
-        if (bd < 0) break;
-        if (bside) hi[bd]++; else lo[bd]--;
-    }
-    return {lo[1], lo[2], lo[3], lo[0], hi[1] - lo[1], hi[2] - lo[2], hi[3] - lo[3], hi[0] - lo[0]};
-}
-
-}  // namespace
-
 void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     const Problem& p = prob_;
     dev::KernelArgs& ka = *(dev::KernelArgs*)out;
@@ -596,7 +545,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
-    std::vector<int> box = choose_box(cnt, (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate, tile_budget_);
+    const TallyBox box = choose_box(cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate, tile_budget_);
     for (int i = 0; i < 8; i++) tile_[i] = box[i];
 }
 
