@@ -103,6 +103,137 @@ def cpu_baseline(target_s: float = 12.0) -> dict:
                       "rate from the binary's own timing window"}
 
 
+def batched_leg(sync) -> dict:
+    """256 uneven jobs (seed 7) in one persistent, graph-replayed launch vs a loop of plain
+    launches of the same jobs. Both trace exactly the same photons (per-job seeds), so their
+    Monte Carlo noise is matched by construction; the noise of the reported per-job quantity
+    (fluence summed over the 5x5x5 voxels under the source) is measured from a second batch with
+    every job's seed changed."""
+    import copy
+
+    import numpy as np
+
+    from umcx_amd import batch as B
+    jobs = B.uneven_jobs(seed=7, njobs=256)
+    photons = sum(j["Session"]["Photons"] for j in jobs)
+    t0 = time.perf_counter()
+    bs = B.BatchSimulation(jobs)
+    setup_s = time.perf_counter() - t0
+    trials = bs.trials()
+    chosen = next(t for t in trials if t["chosen"])
+    sync()
+    t0 = time.perf_counter()
+    batch_ms = bs.run()
+    batch_wall = (time.perf_counter() - t0) * 1e3
+    log(f"bench: batched {photons:.3e} photons in {batch_ms:.1f} ms (tuned: block {chosen['block']} x "
+        f"{chosen['blocks']}, tile {chosen['tile_cells']} cells)")
+    roi_b = np.array([B.roi_sum(bs.fluence(j), jobs[j]) for j in range(len(jobs))])
+    bs.close()
+    jobs2 = copy.deepcopy(jobs)
+    for j in jobs2:
+        j["Session"]["RNGSeed"] = int(j["Session"]["RNGSeed"]) + 1
+    with B.BatchSimulation(jobs2, block=chosen["block"], blocks=chosen["blocks"],
+                           tally_mode=1 if chosen["tile_cells"] else 0) as bs2:
+        bs2.run()
+        roi_2 = np.array([B.roi_sum(bs2.fluence(j), jobs2[j]) for j in range(len(jobs2))])
+    rel = np.abs(roi_b - roi_2) / np.sqrt(2.0) / (0.5 * (roi_b + roi_2))
+    log("bench: plain loop over the same jobs")
+    res, loop_kernel_ms, loop_wall_ms = B.plain_loop(jobs)
+    roi_p = np.array([B.roi_sum(f, c) for (f, _), c in zip(res, jobs)])
+    return {
+        "workload": "256 uneven jobs, seed 7: edges U[32,128], 2-6 box labels, mua U(0.001,0.05), mus U(0.5,20), "
+                    "g U(0.7,0.95), n U(1.33,1.45), Fresnel on, pencil at the z=0 face centre, photons log-U[1e5,1e7], "
+                    "one 0-5 ns gate",
+        "photons": photons, "batch_ms": round(batch_ms, 2), "batch_wall_ms": round(batch_wall, 2),
+        "photons_per_ms": round(photons / batch_ms, 1),
+        "plain_loop_kernel_ms": round(loop_kernel_ms, 2), "plain_loop_wall_ms": round(loop_wall_ms, 2),
+        "speedup_vs_plain_kernel": round(loop_kernel_ms / batch_ms, 3),
+        "speedup_vs_plain_wall": round(loop_wall_ms / batch_wall, 3),
+        "setup_and_tuning_s": round(setup_s, 2),
+        "tuned": {"block": chosen["block"], "blocks": chosen["blocks"], "tile_cells": chosen["tile_cells"],
+                  "trials": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in t.items()} for t in trials]},
+        "reported_quantity": "per-job fluence summed over the 5x5x5 voxels under the source",
+        "noise_rel_std_rms": float(np.sqrt(np.mean(rel ** 2))), "noise_rel_std_max_est": float(rel.max()),
+        "batch_vs_plain_max_rel_diff": float(np.max(np.abs(roi_b - roi_p) / roi_p)),
+    }
+
+
+def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3) -> dict:
+    """d log I_sd / d(mua_v, mus_v) for 16 sources x 16 detectors on the 64x64x32 slab by one
+    forward launch per source plus the replay kernel, against the loop of plain launches that
+    central differences under common random numbers need (2 launches per (source, voxel,
+    parameter) column, each giving the 16 detectors' entries). A mua perturbation leaves every
+    trajectory unchanged under common random numbers, so both estimate the same quantity from
+    the same photons (matched noise); a few columns are computed both ways as a check."""
+    import numpy as np
+
+    from umcx_amd import jacobian as J
+    cfg = J.slab_config(nphoton=photons)
+    srcs = J.grid_sources(64, 64, 4, 16.5)
+    m = J.MultiSourceModel(cfg, srcs, photons)
+    sync()
+    t0 = time.perf_counter()
+    m.forward()
+    fwd_wall = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    Jt = m.jacobian()
+    sync()
+    jac_wall = (time.perf_counter() - t0) * 1e3
+    S, D, nvox = len(srcs), m.ndet, m.nvox
+    entries = S * D * nvox * 2
+    I = m.I.copy()
+    w2 = np.zeros_like(I)
+    for s_, (ids, det, w) in enumerate(m.records):
+        np.add.at(w2[s_], det, w.astype(np.float64) ** 2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        noise = np.sqrt(w2) / I
+    recs = sum(len(r[0]) for r in m.records)
+    log(f"bench: jacobian forward {m.forward_ms:.1f} ms, replay {m.replay_ms:.1f} ms, {recs} records")
+    # the inclusion's voxels carry the largest mua sensitivity; FD columns of source 0 there
+    lab = np.asarray(__import__("umcx_amd").domain(cfg) & 0x7FFFFFFF).reshape(-1)
+    media = cfg["Domain"]["Media"]
+    mua0 = np.array([media[l]["mua"] for l in lab], dtype=np.float32)
+    mus0 = np.array([media[l]["mus"] for l in lab], dtype=np.float32)
+    J0 = Jt[:D, :, 0].float().cpu().numpy()
+    cand = np.argsort(-np.abs(J0).sum(axis=0))[:fd_voxels]
+    fd_ms, fd_wall, err = 0.0, 0.0, []
+    m1 = J.MultiSourceModel(cfg, srcs[:1], photons)
+    for v in cand:
+        h = 1e-3 * float(mua0[v])
+        lp = []
+        for sgn in (1.0, -1.0):
+            mua = mua0.copy()
+            mua[v] = np.float32(mua0[v] + sgn * h)
+            t0 = time.perf_counter()
+            lp.append(m1.forward(mua.reshape(m.shape), mus0.reshape(m.shape))[0])
+            fd_wall += (time.perf_counter() - t0) * 1e3
+            fd_ms += m1.forward_ms
+        fd = (lp[0] - lp[1]) / (2 * h)
+        ok = np.abs(J0[:, v]) > 1e-3 * np.abs(J0[:, v]).max()
+        err.append(float(np.max(np.abs(fd[ok] - J0[ok, v]) / np.abs(J0[ok, v]))))
+    m1.close()
+    m.close()
+    per_launch_ms = fd_ms / (2 * len(cand))
+    per_launch_wall = fd_wall / (2 * len(cand))
+    fd_total_ms = 2 * 2 * nvox * S * per_launch_ms         # 2 launches x 2 parameters per column
+    rep_ms = m.forward_ms + m.replay_ms
+    return {
+        "workload": "64x64x32 slab (mua 0.01, mus 1.0, g 0.9, n 1.37), 6x6x6 inclusion mua 0.035, 16 pencil "
+                    "sources x 16 disk detectors r=1.5 mm on 4x4 grids at z=0, 1e7 photons per source",
+        "entries": entries, "records": recs,
+        "forward_ms": round(m.forward_ms, 2), "replay_ms": round(m.replay_ms, 2),
+        "forward_wall_ms": round(fwd_wall, 2), "jacobian_wall_ms": round(jac_wall, 2),
+        "entries_per_s": entries / (rep_ms * 1e-3),
+        "fd_plain_launch_ms": round(per_launch_ms, 3), "fd_plain_launch_wall_ms": round(per_launch_wall, 3),
+        "fd_loop_ms_full_jacobian": fd_total_ms,
+        "speedup_vs_plain_loop": fd_total_ms / rep_ms,
+        "speedup_vs_plain_loop_wall": (2 * 2 * nvox * S * per_launch_wall) / (fwd_wall + jac_wall),
+        "fd_check_voxels": [int(v) for v in cand], "fd_vs_replay_max_rel_diff": max(err),
+        "reported_quantity": "detected intensity I_sd per source-detector pair",
+        "noise_rel_std_median": float(np.nanmedian(noise)), "noise_rel_std_max": float(np.nanmax(noise)),
+    }
+
+
 def O_domain():
     import umcx_amd as U
     return U.domain(CONFIG)
@@ -115,6 +246,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=PHOTONS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
+    ap.add_argument("--no-jacobian", action="store_true", help="skip the Jacobian-vs-plain leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -168,6 +301,12 @@ def main():
         "detected_per_photon": detected / args.photons,
         "absorbed_percent": (tot - esc) / tot * 100.0,
     }
+    if not args.no_jacobian:
+        log("bench: Jacobian leg")
+        out["jacobian"] = jacobian_leg(sync)
+    if not args.no_batch:
+        log("bench: batched leg")
+        out["batched"] = batched_leg(sync)
     if not args.no_cpu:
         log("bench: CPU baseline")
         try:

@@ -14,6 +14,12 @@ def pytest_configure(config):
 
 
 def has_gpu():
+    """A HIP device is visible to the native library (or, when the library is not built, to torch)."""
+    try:
+        from umcx_amd._native import lib
+        return lib().umcx_device_count() > 0
+    except Exception:
+        pass
     try:
         import torch
         return torch.cuda.is_available()

@@ -82,12 +82,26 @@ class UmcxError(RuntimeError):
     pass
 
 
+def _bind_torch_runtime() -> None:
+    """Load PyTorch's HIP runtime before libumcx.so when PyTorch is installed.
+
+    libumcx.so and PyTorch's ROCm build both need libamdhip64.so.7, and a process holds one
+    library per soname: whichever loads first serves both. PyTorch only works on the runtime
+    it ships with, so it has to come first; libumcx.so runs on either.
+    """
+    try:
+        import torch  # noqa: F401
+    except Exception:
+        pass
+
+
 def lib() -> C.CDLL:
     """Load libumcx.so once; raises when the native library is absent."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise UmcxError(f"native library missing: {LIB_PATH} (build it with `python -m umcx_amd.build`)")
+        _bind_torch_runtime()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -14,8 +14,9 @@
 //
 // Tuning (at creation): a counting pilot of every job gives each job's deposit map, hence its
 // cost per photon and its LDS tile for every candidate tile size; short trial launches over a
-// 1/32 sample of every job then pick the workgroup size, workgroups per CU (so the LDS share
-// and the tile size) and whether to tile at all.
+// 1/32 sample of every job then pick the workgroup size and workgroups per CU (so the LDS share
+// and the tile size). Tiled shapes are preferred over untiled ones for the accuracy of the
+// float32 tallies; tally_mode 0 switches tiles off.
 #pragma once
 #include <cstdint>
 #include <vector>

@@ -178,7 +178,8 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
             bool ended = false;
             if (p.lx > 0.f) {
                 const float4 prop = smedia[p.med & kMed];
-                int nv = step(p, prop.w, g, tabs);
+                uint32_t nlab;
+                int nv = step<uint8_t>(p, prop.w, g, tabs, b.labels + lab_off, nlab);
                 if (nv != p.lastvox) {
                     const uint32_t mcur = p.med & kMed;
                     if (savevol) {
@@ -201,11 +202,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
                     if (p.ly > tend) {
                         ended = true;
                     } else {
-                        uint32_t nm = 0u;
-                        if (nv >= 0) {
-                            const uint32_t l2 = b.labels[lab_off + (uint64_t)nv];
-                            nm = (l2 & 0x7Fu) | ((l2 & 0x80u) << 24);
-                        }
+                        uint32_t nm = (nv >= 0) ? decode_label<uint8_t>(nlab) : 0u;
                         const float4 nprop = smedia[nm & kMed];
                         if (isreflect && prop.w != nprop.w) {
                             if (boundary(p, prop.w, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;
@@ -604,22 +601,22 @@ void BatchSession::tune() {
         return 0.0;
     };
 
-    // 4. trials on a sample of every job; the tile-less variant of the best shape competes too
+    // 4. trials on a sample of every job. Shapes with an LDS tile win over tile-less ones unless
+    // tiles are switched off: a tile also sums each workgroup's hot-cell deposits apart before
+    // one add to HBM, which keeps the float32 tallies of the hottest voxels accurate (a voxel
+    // under a pencil source takes ~1e7 single-precision adds otherwise).
     trials_.clear();
-    auto trial = [&](BatchTrial t) {
+    for (const BatchTrial& c : cands) {
+        BatchTrial t = c;
         t.ms = launch_once(t.block, t.blocks, t.tile_cells, set_of(t.tile_cells), kTrialDiv, kTrialMin, ~0ull, 0);
         t.tile_share = share_of(t.tile_cells);
         trials_.push_back(t);
-    };
-    for (const BatchTrial& c : cands) trial(c);
-    if (lc_.tally_mode < 0) {
-        BatchTrial best = *std::min_element(trials_.begin(), trials_.end(), [](const BatchTrial& a, const BatchTrial& b) { return a.ms < b.ms; });
-        if (best.tile_cells) {
-            best.tile_cells = 0;
-            trial(best);
-        }
     }
-    chosen_ = *std::min_element(trials_.begin(), trials_.end(), [](const BatchTrial& a, const BatchTrial& b) { return a.ms < b.ms; });
+    auto better = [](const BatchTrial& a, const BatchTrial& b) {
+        if ((a.tile_cells > 0) != (b.tile_cells > 0)) return a.tile_cells > 0;
+        return a.ms < b.ms;
+    };
+    chosen_ = *std::min_element(trials_.begin(), trials_.end(), better);
     if (chosen_.tile_cells) boxes_ = *set_of(chosen_.tile_cells);
     else boxes_.assign(probs_.size(), TallyBox{0, 0, 0, 0, 0, 0, 0, 0});
 }

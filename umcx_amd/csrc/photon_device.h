@@ -72,6 +72,18 @@ struct Photon {
     float cmua, cmus;          // absorption and scattering coefficients of lastvox
 };
 
+// raw label loads (issued early, decoded where used) and their decoding into the kernels' form:
+// medium in the low bits, detector flag in bit 31
+template <typename LabelT>
+__device__ __forceinline__ uint32_t raw_label(const void* labels, int idx) {
+    return ((const LabelT*)labels)[idx];
+}
+template <typename LabelT>
+__device__ __forceinline__ uint32_t decode_label(uint32_t v) {
+    if (sizeof(LabelT) == 1) return (v & 0x7Fu) | ((v & 0x80u) << 24);
+    return v;
+}
+
 template <typename LabelT>
 __device__ __forceinline__ uint32_t load_label(const void* labels, int idx);
 template <>
@@ -186,14 +198,26 @@ __device__ __forceinline__ bool skip_to_box(Photon& p, const Grid& a) {
     return true;
 }
 
-// step (:315-338); returns the voxel the photon is in afterwards (-1 outside)
-__device__ __forceinline__ int step(Photon& p, float n, const Grid& a, const Tabs& t) {
+// step (:315-338); returns the voxel the photon is in afterwards (-1 outside). The label of the
+// voxel across the nearest face is loaded before the weight update and handed back raw in
+// `nlab`, so its memory latency overlaps the step's arithmetic and the deposit; it is valid
+// whenever the photon crosses into a voxel inside the grid.
+template <typename LabelT>
+__device__ __forceinline__ int step(Photon& p, float n, const Grid& a, const Tabs& t, const void* labels,
+                                    uint32_t& nlab) {
     const float mua = p.cmua, mus = p.cmus;
     float hx = fabsf(((float)(p.ix + (p.vx > 0.f)) - p.px) * p.rx);
     float hy = fabsf(((float)(p.iy + (p.vy > 0.f)) - p.py) * p.ry);
     float hz = fabsf(((float)(p.iz + (p.vz > 0.f)) - p.pz) * p.rz);
     float dmin = fminf(fminf(hx, hy), hz);
     p.iw = (dmin == hx) ? 0 : ((dmin == hy) ? 1 : 2);
+    // the voxel across the nearest face (value selects keep the photon in registers)
+    const int ax = p.iw;
+    const int cx = (int)(short)(p.ix + (ax == 0 ? (p.vx > 0.f ? 1 : -1) : 0));
+    const int cy = (int)(short)(p.iy + (ax == 1 ? (p.vy > 0.f ? 1 : -1) : 0));
+    const int cz = (int)(short)(p.iz + (ax == 2 ? (p.vz > 0.f ? 1 : -1) : 0));
+    const int cand = vox_index(a, cx, cy, cz);
+    nlab = raw_label<LabelT>(labels, cand >= 0 ? cand : 0);
     float used = fminf(dmin * mus, p.lx);
     bool cont = used != p.lx;
     float d = (mus == 0.f) ? dmin : (used / mus);
@@ -205,12 +229,10 @@ __device__ __forceinline__ int step(Photon& p, float n, const Grid& a, const Tab
     p.ly += d * n * kOneOverC0;
     p.lz += d;
     if (cont) {
-        // value selects with unconditional writes keep the photon in registers
-        const int ax = p.iw;
-        p.ix = (int)(short)(p.ix + (ax == 0 ? (p.vx > 0.f ? 1 : -1) : 0));
-        p.iy = (int)(short)(p.iy + (ax == 1 ? (p.vy > 0.f ? 1 : -1) : 0));
-        p.iz = (int)(short)(p.iz + (ax == 2 ? (p.vz > 0.f ? 1 : -1) : 0));
-        return vox_index(a, p.ix, p.iy, p.iz);
+        p.ix = cx;
+        p.iy = cy;
+        p.iz = cz;
+        return cand;
     }
     return p.lastvox;
 }

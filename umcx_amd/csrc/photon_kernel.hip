@@ -182,7 +182,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         // ---- one step of sprint (:282-311) ----
         if (p.lx > 0.f && (a.vote == 0 || !scatter_round)) {
             const float4 prop = smedia[p.med & kMed];
-            int nv = step(p, prop.w, grid, tabs);
+            uint32_t nlab;
+            int nv = step<LabelT>(p, prop.w, grid, tabs, a.labels, nlab);
             if (nv != p.lastvox) {
                 const uint32_t m = p.med & kMed;
                 if (REPLAY) {
@@ -215,7 +216,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 if (p.ly > a.tend) {
                     ended = true;
                 } else {
-                    uint32_t nm = (nv >= 0) ? load_label<LabelT>(a.labels, nv) : 0u;
+                    uint32_t nm = (nv >= 0) ? decode_label<LabelT>(nlab) : 0u;
                     const float4 nprop = smedia[nm & kMed];
                     if (REFLECT && prop.w != nprop.w) {
                         if (boundary(p, prop.w, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;

@@ -195,6 +195,28 @@ def log_intensity(mua, mus, model: MultiSourceModel):
     return _FN.apply(mua, mus, model)
 
 
+def slab_config(seed: int = 11, nphoton: int = 10_000_000, inclusion_mua: float = 0.035) -> dict:
+    """The differentiable workload: a 64x64x32 slab of 1 mm voxels (mua 0.01/mm, mus 1.0/mm,
+    g 0.9, n 1.37) with one 6x6x6 inclusion (label 2, mua ``inclusion_mua``) at a seeded
+    position, and 16 disk detectors of radius 1.5 mm on a 4x4 grid (pitch 8 mm from (20, 20)) on
+    the z=0 face. Sources come from ``grid_sources(64, 64, 4, 16.5)``."""
+    rng = np.random.default_rng(seed)
+    vol = np.ones((64, 64, 32), dtype=np.int32)
+    x0, y0, z0 = (int(rng.integers(8, 64 - 14)), int(rng.integers(8, 64 - 14)), int(rng.integers(4, 32 - 10)))
+    vol[x0:x0 + 6, y0:y0 + 6, z0:z0 + 6] = 2
+    return {"Session": {"ID": "slab", "Photons": int(nphoton), "RNGSeed": 1648335518, "DoMismatch": 1,
+                        "DoPartialPath": 1, "SaveDetFlag": 5, "OutputType": "x"},
+            "Forward": {"T0": 0, "T1": 5e-9, "Dt": 5e-9},
+            "Domain": {"Dim": [64, 64, 32], "LengthUnit": 1,
+                       "Media": [{"mua": 0, "mus": 0, "g": 1, "n": 1},
+                                 {"mua": 0.01, "mus": 1.0, "g": 0.9, "n": 1.37},
+                                 {"mua": float(inclusion_mua), "mus": 1.0, "g": 0.9, "n": 1.37}]},
+            "Optode": {"Source": {"Type": "pencil", "Pos": [24.5, 24.5, 0], "Dir": [0, 0, 1]},
+                       "Detector": [{"Pos": [20 + 8 * i, 20 + 8 * j, 0], "R": 1.5} for j in range(4) for i in range(4)]},
+            "Shapes": {"_ArraySize_": [64, 64, 32], "_ArrayType_": "int32",
+                       "_ArrayData_": vol.reshape(-1).tolist()}}
+
+
 def grid_sources(nx, ny, n_side, margin, z=0.0, offset=0.0):
     """Pencil sources (+z) on an n_side x n_side grid inside [margin, n-margin]."""
     xs = np.linspace(margin, nx - margin, n_side) + offset
