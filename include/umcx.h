@@ -44,6 +44,8 @@ typedef struct umcx_launch {
     int32_t tally_mode;  /* -1 tuned, 0 global atomics, 1 LDS tile */
     int32_t trace;       /* 1: keep every photon's final state (small runs) */
     int32_t vote;        /* scatter-round threshold in lanes; 0 scatters as soon as a path ends, -1 tuned */
+    int32_t label_cache; /* 0: media ids packed into LDS when the volume fits (<= 96 KiB at 2 or 4 bits
+                            per voxel), -1: labels always read from HBM */
 } umcx_launch;
 
 /* Constants of a prepared problem: MCX_param (src/umcx.cpp:95-99) plus launch facts. */
@@ -55,6 +57,7 @@ typedef struct umcx_info {
     float tstart, tend, rtstep, unitinmm;
     uint32_t seeds[4];
     int32_t block, blocks, tally_mode, label_bytes;
+    int32_t lds_label_bits;  /* bits per voxel of the LDS-staged labels (0: labels read from HBM) */
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)

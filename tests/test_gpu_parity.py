@@ -47,13 +47,16 @@ def test_device_libm_bitwise_on_reachable_inputs():
         assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32)), f"fn {fn}"
 
 
+@pytest.mark.parametrize("label_cache", [0, -1])
 @pytest.mark.parametrize("name", TRACE_CASES)
-def test_per_photon_state_matches_oracle(name):
+def test_per_photon_state_matches_oracle(name, label_cache):
+    """label_cache 0: media ids staged in LDS where the volume fits; -1: labels read from HBM."""
     g, prob = O.golden_problem(name)
     n = 3000 if name not in ("c3", "skinvessel") else 1500
-    say(f"trace parity {name} n={n}")
+    say(f"trace parity {name} n={n} label_cache {label_cache}")
     o = O.run(prob, nphoton=n, traces=True, nthreads=8)["traces"]
-    with U.Simulation(g["cfg"], trace=True, vote=32 if name in ("c2", "cube60b", "c3") else 0) as sim:
+    with U.Simulation(g["cfg"], trace=True, vote=32 if name in ("c2", "cube60b", "c3") else 0,
+                      label_cache=label_cache) as sim:
         sim.run(0, n)
         t = sim.trace(n)
     exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
@@ -67,11 +70,12 @@ def test_per_photon_state_matches_oracle(name):
     assert np.array_equal(exact(t["len"][:, 3]), exact(o["len"][:, 3]))
 
 
+@pytest.mark.parametrize("label_cache", [0, -1])
 @pytest.mark.parametrize("name", DETP_CASES)
-def test_detected_rows_match_reference(name):
+def test_detected_rows_match_reference(name, label_cache):
     g = O.golden(name)
-    say(f"detp parity {name}")
-    with U.Simulation(g["cfg"]) as sim:
+    say(f"detp parity {name} label_cache {label_cache}")
+    with U.Simulation(g["cfg"], label_cache=label_cache) as sim:
         sim.run()
         rows, ids, w = sim.detp()
         detected, saved = sim.detected()
@@ -80,6 +84,55 @@ def test_detected_rows_match_reference(name):
     assert got.shape == g["detp"].shape
     assert np.array_equal(got.view(np.uint32), g["detp"].view(np.uint32))
     assert len(np.unique(ids)) >= 1 and ids.max() < int(float(g["cfg"]["Session"]["Photons"]))
+
+
+def test_label_staging_choice():
+    """Volumes whose media ids pack into <= 96 KiB are staged in LDS (2 bits for <= 4 media, 4 bits
+    for <= 16); larger ones, and label_cache -1, read labels from HBM."""
+    expect = {"c2": 2, "c4src": 2, "cube60": 2, "zipvol": 2, "sphshells": 0, "c3": 0}
+    for name, bits in expect.items():
+        g = O.golden(name)
+        with U.Simulation(g["cfg"]) as sim:
+            assert sim.info["lds_label_bits"] == bits, name
+        with U.Simulation(g["cfg"], label_cache=-1) as sim:
+            assert sim.info["lds_label_bits"] == 0, name
+
+
+def layered_config(nphoton=2000):
+    """40^3 voxels in five layers of different media and refractive index (4-bit LDS labels),
+    Fresnel reflection on, two detectors."""
+    return {"Session": {"ID": "layers5", "Photons": nphoton, "RNGSeed": 29012, "DoMismatch": 1, "SaveDetFlag": 5},
+            "Forward": {"T0": 0, "T1": 5e-9, "Dt": 5e-10},
+            "Domain": {"Dim": [40, 40, 40], "LengthUnit": 1,
+                       "Media": [{"mua": 0, "mus": 0, "g": 1, "n": 1},
+                                 {"mua": 0.01, "mus": 2.0, "g": 0.8, "n": 1.33},
+                                 {"mua": 0.02, "mus": 5.0, "g": 0.9, "n": 1.45},
+                                 {"mua": 0.005, "mus": 1.0, "g": 0.7, "n": 1.37},
+                                 {"mua": 0.05, "mus": 10.0, "g": 0.95, "n": 1.40},
+                                 {"mua": 0.001, "mus": 0.5, "g": 0.5, "n": 1.50}]},
+            "Optode": {"Source": {"Type": "pencil", "Pos": [20, 20, 0], "Dir": [0, 0, 1]},
+                       "Detector": [{"Pos": [14, 20, 0], "R": 2}, {"Pos": [26, 20, 0], "R": 2}]},
+            "Shapes": [{"Grid": {"Tag": 1, "Size": [40, 40, 40]}},
+                       {"ZLayers": [[8, 16, 2], [16, 24, 3], [24, 32, 4], [32, 40, 5]]}]}
+
+
+def test_four_bit_lds_labels_match_oracle():
+    cfg = layered_config()
+    prob = O.problem_from_config(cfg, U.domain(cfg))
+    n = cfg["Session"]["Photons"]
+    say("trace parity layered 5 media (4-bit LDS labels)")
+    o = O.run(prob, nphoton=n, traces=True, nthreads=8)
+    with U.Simulation(cfg, trace=True) as sim:
+        assert sim.info["lds_label_bits"] == 4
+        sim.run(0, n)
+        t = sim.trace(n)
+        rows = sorted_rows(sim.detp()[0])
+    ot = o["traces"]
+    exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
+    for key in ("pos", "vec", "len"):
+        assert np.array_equal(exact(t[key]), exact(ot[key])), key
+    assert np.array_equal(t["mediaid"], ot["mediaid"])
+    assert np.array_equal(rows.view(np.uint32), sorted_rows(o["detp"]).view(np.uint32))
 
 
 def test_detector_overflow_keeps_genuine_rows():
@@ -180,8 +233,9 @@ def test_photon_ranges_compose():
 def test_launch_shape_does_not_change_results():
     g = O.golden("cube60b")
     outs = []
-    for block, blocks, vote in ((64, 7, 0), (256, 0, 0), (128, 33, 0), (256, 0, 24), (256, 0, 48), (64, 5, 64)):
-        with U.Simulation(g["cfg"], block=block, blocks=blocks, vote=vote) as sim:
+    for block, blocks, vote, lc in ((64, 7, 0, 0), (256, 0, 0, -1), (128, 33, 0, 0), (256, 0, 24, -1), (256, 0, 48, 0),
+                                    (64, 5, 64, 0), (1024, 0, 0, 0), (512, 3, 0, 0)):
+        with U.Simulation(g["cfg"], block=block, blocks=blocks, vote=vote, label_cache=lc) as sim:
             sim.run()
             outs.append((sorted_rows(sim.detp()[0]), sim.energies()))
     for rows, e in outs[1:]:

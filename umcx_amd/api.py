@@ -50,9 +50,9 @@ class Simulation:
     """One prepared problem on the GPU."""
 
     def __init__(self, cfg=None, argv: list[str] | None = None, device: int = 0, block: int = 0,
-                 blocks: int = 0, tally_mode: int = -1, trace: bool = False, vote: int = -1):
+                 blocks: int = 0, tally_mode: int = -1, trace: bool = False, vote: int = -1, label_cache: int = 0):
         L = lib()
-        launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote)
+        launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote, label_cache)
         h = C.c_void_p()
         if argv is not None:
             args = ["umcx"] + list(argv)

@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
             if (p.lx > 0.f) {
                 const float4 prop = smedia[p.med & kMed];
                 uint32_t nlab;
-                int nv = step<uint8_t>(p, prop.w, g, tabs, b.labels + lab_off, nlab);
+                int nv = step(p, prop.w, g, tabs, GlobalLabels<uint8_t>{b.labels + lab_off}, nlab);
                 if (nv != p.lastvox) {
                     const uint32_t mcur = p.med & kMed;
                     if (savevol) {
@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
                     if (p.ly > tend) {
                         ended = true;
                     } else {
-                        uint32_t nm = (nv >= 0) ? decode_label<uint8_t>(nlab) : 0u;
+                        uint32_t nm = (nv >= 0) ? GlobalLabels<uint8_t>{nullptr}.decode(nlab) : 0u;
                         const float4 nprop = smedia[nm & kMed];
                         if (isreflect && prop.w != nprop.w) {
                             if (boundary(p, prop.w, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;

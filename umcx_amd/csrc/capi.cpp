@@ -52,6 +52,7 @@ umcx::LaunchConfig to_launch(const umcx_launch* l) {
         lc.tally_mode = l->tally_mode;
         lc.trace = l->trace != 0;
         lc.vote = l->vote;
+        lc.label_cache = l->label_cache;
     }
     return lc;
 }
@@ -194,6 +195,7 @@ int umcx_session_info(const umcx_session* s, umcx_info* info) {
         for (int i = 0; i < 4; i++) info->seeds[i] = p.seeds[i];
         info->block = s->s->block(); info->blocks = s->s->blocks(); info->tally_mode = s->s->tally_mode();
         info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
+        info->lds_label_bits = s->s->lds_label_bits();
         return 0;
     });
 }

@@ -72,17 +72,30 @@ struct Photon {
     float cmua, cmus;          // absorption and scattering coefficients of lastvox
 };
 
-// raw label loads (issued early, decoded where used) and their decoding into the kernels' form:
-// medium in the low bits, detector flag in bit 31
+// Label sources. raw() issues the load (its value is waited for where decode() uses it), decode()
+// gives the kernels' form: medium in the low bits, detector flag in bit 31.
+// GlobalLabels: the label volume in HBM (uint8 with the detector flag in bit 7, or uint32).
 template <typename LabelT>
-__device__ __forceinline__ uint32_t raw_label(const void* labels, int idx) {
-    return ((const LabelT*)labels)[idx];
-}
-template <typename LabelT>
-__device__ __forceinline__ uint32_t decode_label(uint32_t v) {
-    if (sizeof(LabelT) == 1) return (v & 0x7Fu) | ((v & 0x80u) << 24);
-    return v;
-}
+struct GlobalLabels {
+    const LabelT* p;
+    __device__ __forceinline__ uint32_t raw(int idx) const { return p[idx]; }
+    __device__ __forceinline__ uint32_t decode(uint32_t v) const {
+        if (sizeof(LabelT) == 1) return (v & 0x7Fu) | ((v & 0x80u) << 24);
+        return v;
+    }
+};
+// LdsLabels: media ids packed 2 or 4 bits per voxel into 32-bit words staged in LDS (no detector
+// flag: the kernels read that one from HBM when a photon ends).
+struct LdsLabels {
+    const uint32_t* w;
+    uint32_t lg;      // log2(voxels per word): 4 for 2-bit, 3 for 4-bit labels
+    uint32_t bits;    // 2 or 4
+    __device__ __forceinline__ uint32_t raw(int idx) const {
+        const uint32_t i = (uint32_t)idx;
+        return (w[i >> lg] >> ((i & ((1u << lg) - 1u)) * bits)) & ((1u << bits) - 1u);
+    }
+    __device__ __forceinline__ uint32_t decode(uint32_t v) const { return v; }
+};
 
 template <typename LabelT>
 __device__ __forceinline__ uint32_t load_label(const void* labels, int idx);
@@ -202,8 +215,8 @@ __device__ __forceinline__ bool skip_to_box(Photon& p, const Grid& a) {
 // voxel across the nearest face is loaded before the weight update and handed back raw in
 // `nlab`, so its memory latency overlaps the step's arithmetic and the deposit; it is valid
 // whenever the photon crosses into a voxel inside the grid.
-template <typename LabelT>
-__device__ __forceinline__ int step(Photon& p, float n, const Grid& a, const Tabs& t, const void* labels,
+template <typename Labels>
+__device__ __forceinline__ int step(Photon& p, float n, const Grid& a, const Tabs& t, const Labels& labels,
                                     uint32_t& nlab) {
     const float mua = p.cmua, mus = p.cmus;
     float hx = fabsf(((float)(p.ix + (p.vx > 0.f)) - p.px) * p.rx);
@@ -217,7 +230,7 @@ __device__ __forceinline__ int step(Photon& p, float n, const Grid& a, const Tab
     const int cy = (int)(short)(p.iy + (ax == 1 ? (p.vy > 0.f ? 1 : -1) : 0));
     const int cz = (int)(short)(p.iz + (ax == 2 ? (p.vz > 0.f ? 1 : -1) : 0));
     const int cand = vox_index(a, cx, cy, cz);
-    nlab = raw_label<LabelT>(labels, cand >= 0 ? cand : 0);
+    nlab = labels.raw(cand >= 0 ? cand : 0);
     float used = fminf(dmin * mus, p.lx);
     bool cont = used != p.lx;
     float d = (mus == 0.f) ? dmin : (used / mus);

@@ -33,6 +33,10 @@
     } while (0)
 
 namespace umcx {
+
+constexpr uint64_t kLabelCacheBytes = 96 * 1024;   // largest packed label volume staged in LDS
+constexpr uint64_t kMaxTileCells = 32768;          // largest LDS tally tile
+
 namespace dev {
 
 struct KernelArgs {
@@ -72,6 +76,25 @@ struct KernelArgs {
     uint64_t gstride;
     float unit;              // mm per voxel
     float* rw;               // final weight of each replayed photon (may be null)
+    // media ids packed plabel_bits (2 or 4) per voxel, staged in LDS (LDSLAB kernels)
+    const uint32_t* plabels;
+    uint32_t plabel_words, plabel_lg, plabel_bits;
+    // photon slots: workgroup w first takes [w*per_wg, (w+1)*per_wg) through an LDS counter,
+    // then slots from blocks*per_wg on through the device cursor
+    uint64_t per_wg;
+};
+
+template <bool LDSLAB, typename LabelT>
+struct LabelPick {
+    using T = GlobalLabels<LabelT>;
+    static __device__ __forceinline__ T make(const KernelArgs& a, const uint32_t*) { return T{(const LabelT*)a.labels}; }
+};
+template <typename LabelT>
+struct LabelPick<true, LabelT> {
+    using T = LdsLabels;
+    static __device__ __forceinline__ T make(const KernelArgs& a, const uint32_t* s) {
+        return T{s, a.plabel_lg, a.plabel_bits};
+    }
 };
 
 __device__ __forceinline__ int tile_local(const KernelArgs& a, int ix, int iy, int iz) {
@@ -94,26 +117,37 @@ __device__ __forceinline__ void load_props(Photon& p, const KernelArgs& a, const
 
 // REPLAY: re-trace the photons listed in a.rids (photon i is a pure function of i) and add
 // each voxel segment's perturbation-MC terms to the gradient slices instead of tallying.
-template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY>
+template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY, bool LDSLAB>
 __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
-    // LDS: libm tables | media | tally tile | per-lane partial paths
+    // LDS: libm tables | media | packed labels (LDSLAB) | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
+    __shared__ uint32_t s_taken;                                 // static photon slots handed out
     libm::LogfEntry* slog = (libm::LogfEntry*)lds4;            // 16 x 16 B
     uint64_t* sexp = (uint64_t*)(lds4 + 16);                     // 32 x 8 B
     float4* smedia = lds4 + 32;
-    float* stile = (float*)(smedia + a.mediumnum);
+    uint32_t* slab = (uint32_t*)(smedia + a.mediumnum);
+    float* stile = (float*)(slab + (LDSLAB ? a.plabel_words : 0u));
     float* sppath = stile + a.tile_n;
     const int tid = threadIdx.x, bd = blockDim.x;
     if (tid < 16) slog[tid] = libm::kLogfTab[tid];
     if (tid < 32) sexp[tid] = libm::kExp2Tab[tid];
+    if (tid == 0) s_taken = 0u;
     const Tabs tabs{slog, sexp};
     const Grid grid{a.nx, a.ny, a.nz, a.nyz};
     const Source src{a.srcpos, a.srcdir, a.srcp1, a.srcp2, a.srctype};
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
+    if (LDSLAB)
+        for (uint32_t i = tid; i < a.plabel_words; i += bd) slab[i] = a.plabels[i];
     for (int i = tid; i < (int)a.tile_n; i += bd) stile[i] = 0.f;
     if (SAVEDET)
         for (int i = tid; i < a.ppathlen * bd; i += bd) sppath[i] = 0.f;
     __syncthreads();
+    const auto labs = LabelPick<LDSLAB, LabelT>::make(a, slab);
+    // detector flag of the voxel a photon ends in (LDS labels carry media ids only)
+    auto det_flag = [&](const Photon& q) -> uint32_t {
+        if (LDSLAB) return load_label<LabelT>(a.labels, q.lastvox) & kDet;
+        return q.med & kDet;
+    };
 
     Photon p;
     Rng r;
@@ -128,13 +162,24 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
 
     while (true) {
         // ---- take a new photon when this lane's has ended (one atomic per wave) ----
+        // The workgroup's static slots come through an LDS counter, so a refill does not wait on
+        // a device atomic (whose wait would also cover every fluence atomic still in flight).
         if (!alive) {
             const unsigned long long need = __ballot(1);
             const int leader = __ffsll((long long)need) - 1;
-            unsigned long long base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(a.cursor, (unsigned long long)__popcll(need));
-            base = __shfl(base, leader);
-            const unsigned long long k = base + (unsigned long long)__popcll(need & lanemask_lt());
+            const uint32_t n = (uint32_t)__popcll(need), rank = (uint32_t)__popcll(need & lanemask_lt());
+            uint32_t sb = 0, nstat = 0;
+            unsigned long long gb = 0;
+            if ((int)__lane_id() == leader) {
+                sb = atomicAdd(&s_taken, n);
+                nstat = sb < a.per_wg ? (uint32_t)min((uint64_t)n, a.per_wg - sb) : 0u;
+                if (nstat < n) gb = atomicAdd(a.cursor, (unsigned long long)(n - nstat));
+            }
+            sb = __shfl(sb, leader);
+            nstat = __shfl(nstat, leader);
+            gb = __shfl(gb, leader);
+            const unsigned long long k = rank < nstat ? (uint64_t)blockIdx.x * a.per_wg + sb + rank
+                                                      : (uint64_t)gridDim.x * a.per_wg + gb + (rank - nstat);
             if (k >= a.count) break;
             pid = REPLAY ? (uint64_t)a.rids[k] : a.first + k;
             if (REPLAY) {
@@ -165,7 +210,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 continue;
             }
             p.tvox = tile_local(a, p.ix, p.iy, p.iz);
-            p.med = load_label<LabelT>(a.labels, p.lastvox);                                         // :269
+            p.med = labs.decode(labs.raw(p.lastvox));                                                // :269
             load_props(p, a, smedia);
             p.lx = rng_scatlen(r, tabs);                                                             // :270
             alive = true;
@@ -183,7 +228,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         if (p.lx > 0.f && (a.vote == 0 || !scatter_round)) {
             const float4 prop = smedia[p.med & kMed];
             uint32_t nlab;
-            int nv = step<LabelT>(p, prop.w, grid, tabs, a.labels, nlab);
+            int nv = step(p, prop.w, grid, tabs, labs, nlab);
             if (nv != p.lastvox) {
                 const uint32_t m = p.med & kMed;
                 if (REPLAY) {
@@ -216,7 +261,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 if (p.ly > a.tend) {
                     ended = true;
                 } else {
-                    uint32_t nm = (nv >= 0) ? decode_label<LabelT>(nlab) : 0u;
+                    uint32_t nm = (nv >= 0) ? labs.decode(nlab) : 0u;
                     const float4 nprop = smedia[nm & kMed];
                     if (REFLECT && prop.w != nprop.w) {
                         if (boundary(p, prop.w, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;
@@ -249,7 +294,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             continue;
         }
         // ---- photon ends: detection (:276-278) and energy tallies (:765-770) ----
-        if (SAVEDET && (p.med & kDet) && p.ly <= a.tend) {
+        if (SAVEDET && p.ly <= a.tend && det_flag(p)) {
             for (int d = 0; d < a.detnum; d++) {
                 const float4 dp = a.detpos[d];
                 float dx = dp.x - p.px, dy = dp.y - p.py, dz = dp.z - p.pz;
@@ -278,7 +323,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             t.pos[0] = p.px; t.pos[1] = p.py; t.pos[2] = p.pz; t.pos[3] = p.pw;
             t.vec[0] = p.vx; t.vec[1] = p.vy; t.vec[2] = p.vz; t.vec[3] = p.vw;
             t.len[0] = p.lx; t.len[1] = p.ly; t.len[2] = p.lz; t.len[3] = p.lw;
-            t.mediaid = (int32_t)p.med; t.lastvoxelidx = p.lastvox; t.w0 = w0;
+            t.mediaid = (int32_t)((p.med & kMed) | det_flag(p)); t.lastvoxelidx = p.lastvox; t.w0 = w0;
         }
         alive = false;
     }
@@ -343,32 +388,46 @@ namespace {
 
 using LaunchFn = void (*)(const dev::KernelArgs&, int, int, size_t, hipStream_t);
 
-template <bool R, bool D, typename L, bool P = false>
+template <bool R, bool D, typename L, bool P = false, bool S = false>
 void launch_one(const dev::KernelArgs& ka, int blocks, int block, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((dev::photon_kernel<R, D, L, P>), dim3(blocks), dim3(block), lds, s, ka);
+    hipLaunchKernelGGL((dev::photon_kernel<R, D, L, P, S>), dim3(blocks), dim3(block), lds, s, ka);
 }
 
-LaunchFn pick_replay(bool reflect, bool u8) {
-    if (u8) return reflect ? launch_one<true, false, uint8_t, true> : launch_one<false, false, uint8_t, true>;
-    return reflect ? launch_one<true, false, uint32_t, true> : launch_one<false, false, uint32_t, true>;
-}
-
-LaunchFn pick(bool reflect, bool savedet, bool u8) {
-    if (u8) {
-        if (reflect) return savedet ? launch_one<true, true, uint8_t> : launch_one<true, false, uint8_t>;
-        return savedet ? launch_one<false, true, uint8_t> : launch_one<false, false, uint8_t>;
+// kernel variants: reflection, detectors, label type, replay, labels in LDS (uint8 labels only)
+template <bool P>
+LaunchFn pick_variant(bool reflect, bool savedet, bool u8, bool lds) {
+    if (u8 && lds) {
+        if (reflect) return savedet ? launch_one<true, true, uint8_t, P, true> : launch_one<true, false, uint8_t, P, true>;
+        return savedet ? launch_one<false, true, uint8_t, P, true> : launch_one<false, false, uint8_t, P, true>;
     }
-    if (reflect) return savedet ? launch_one<true, true, uint32_t> : launch_one<true, false, uint32_t>;
-    return savedet ? launch_one<false, true, uint32_t> : launch_one<false, false, uint32_t>;
+    if (u8) {
+        if (reflect) return savedet ? launch_one<true, true, uint8_t, P> : launch_one<true, false, uint8_t, P>;
+        return savedet ? launch_one<false, true, uint8_t, P> : launch_one<false, false, uint8_t, P>;
+    }
+    if (reflect) return savedet ? launch_one<true, true, uint32_t, P> : launch_one<true, false, uint32_t, P>;
+    return savedet ? launch_one<false, true, uint32_t, P> : launch_one<false, false, uint32_t, P>;
 }
 
-const void* kernel_ptr(bool reflect, bool savedet, bool u8) {
-    if (u8) {
-        if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint8_t, false> : (const void*)dev::photon_kernel<true, false, uint8_t, false>;
-        return savedet ? (const void*)dev::photon_kernel<false, true, uint8_t, false> : (const void*)dev::photon_kernel<false, false, uint8_t, false>;
+LaunchFn pick_replay(bool reflect, bool u8, bool lds) { return pick_variant<true>(reflect, false, u8, lds); }
+
+LaunchFn pick(bool reflect, bool savedet, bool u8, bool lds) { return pick_variant<false>(reflect, savedet, u8, lds); }
+
+template <bool R, bool D, typename L, bool S>
+const void* kptr() {
+    return (const void*)dev::photon_kernel<R, D, L, false, S>;
+}
+
+const void* kernel_ptr(bool reflect, bool savedet, bool u8, bool lds) {
+    if (u8 && lds) {
+        if (reflect) return savedet ? kptr<true, true, uint8_t, true>() : kptr<true, false, uint8_t, true>();
+        return savedet ? kptr<false, true, uint8_t, true>() : kptr<false, false, uint8_t, true>();
     }
-    if (reflect) return savedet ? (const void*)dev::photon_kernel<true, true, uint32_t, false> : (const void*)dev::photon_kernel<true, false, uint32_t, false>;
-    return savedet ? (const void*)dev::photon_kernel<false, true, uint32_t, false> : (const void*)dev::photon_kernel<false, false, uint32_t, false>;
+    if (u8) {
+        if (reflect) return savedet ? kptr<true, true, uint8_t, false>() : kptr<true, false, uint8_t, false>();
+        return savedet ? kptr<false, true, uint8_t, false>() : kptr<false, false, uint8_t, false>();
+    }
+    if (reflect) return savedet ? kptr<true, true, uint32_t, false>() : kptr<true, false, uint32_t, false>();
+    return savedet ? kptr<false, true, uint32_t, false>() : kptr<false, false, uint32_t, false>();
 }
 
 template <typename T>
@@ -415,6 +474,19 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
         }
         d_labels_ = dalloc<uint8_t>(nvox);
         HIP_OK(hipMemcpy(d_labels_, lab.data(), nvox, hipMemcpyHostToDevice));
+        // media ids packed 2 or 4 bits per voxel for LDS staging, when the whole volume fits
+        const uint32_t bits = p.mediumnum <= 4 ? 2u : (p.mediumnum <= 16 ? 4u : 0u);
+        const uint64_t words = bits ? (nvox * bits + 31) / 32 : 0;
+        if (lc.label_cache >= 0 && bits && words * 4 <= kLabelCacheBytes) {
+            std::vector<uint32_t> packed(words, 0u);
+            const uint32_t per = 32 / bits;
+            for (uint64_t i = 0; i < nvox; i++)
+                packed[i / per] |= ((uint32_t)p.labels[i] & kMedMask) << ((i % per) * bits);
+            d_plabels_ = dalloc<uint32_t>(words);
+            HIP_OK(hipMemcpy(d_plabels_, packed.data(), words * 4, hipMemcpyHostToDevice));
+            plabel_words_ = (uint32_t)words;
+            plabel_bits_ = bits;
+        }
     } else {
         d_labels_ = dalloc<uint32_t>(nvox);
         HIP_OK(hipMemcpy(d_labels_, p.labels.data(), nvox * 4, hipMemcpyHostToDevice));
@@ -437,10 +509,11 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     // launch shape: fill every CU with persistent waves
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, lc.device));
-    block_ = lc.block > 0 ? lc.block : (p.blocksize > 0 ? p.blocksize : 256);
+    // with the labels in LDS, one 1024-lane workgroup per CU holds one copy of them
+    block_ = lc.block > 0 ? lc.block : (plabel_words_ ? 1024 : (p.blocksize > 0 ? p.blocksize : 256));
     block_ = std::max(64, std::min(1024, (block_ / 64) * 64));
     size_t lds = lds_bytes(0);
-    const size_t lds_wg_max = std::min<size_t>(prop.sharedMemPerBlock, 64 * 1024);
+    const size_t lds_wg_max = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 64 * 1024;
     while (lds > lds_wg_max && block_ > 64) {
         block_ /= 2;
         lds = lds_bytes(0);
@@ -448,13 +521,13 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     if (lds > lds_wg_max) throw std::runtime_error("too many media for the LDS staging of media and partial paths");
     int per_cu = 0;
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1), block_, lds));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0), block_, lds));
     if (per_cu < 1) per_cu = 1;
     // LDS tally tile: share the CU's LDS between the workgroups the register budget lets in
     tally_mode_ = lc.tally_mode;
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 160 * 1024;
     size_t per_wg = std::min(lds_cu / (size_t)per_cu, lds_wg_max);
-    tile_budget_ = per_wg > lds ? std::min<uint64_t>((per_wg - lds) / 4, 8192) : 0;
+    tile_budget_ = per_wg > lds ? std::min<uint64_t>((per_wg - lds) / 4, kMaxTileCells) : 0;
     if (tally_mode_ == 1 && tile_budget_ < 64) tile_budget_ = 64;
     blocks_ = lc.blocks > 0 ? lc.blocks : prop.multiProcessorCount * per_cu;
     if (p.threadnum > 0 && lc.blocks <= 0) blocks_ = std::max(1, p.threadnum / block_);
@@ -465,7 +538,7 @@ Session::~Session() {
     (void)hipSetDevice(lc_.device);
     for (void* ptr : {(void*)d_labels_, (void*)d_media_, (void*)d_detpos_, (void*)d_fluence_, (void*)d_detp_,
                       (void*)d_detid_, (void*)d_detw_, (void*)d_counters_, (void*)d_cursor_, (void*)d_energy_,
-                      (void*)d_trace_, d_vprops_})
+                      (void*)d_trace_, d_vprops_, (void*)d_plabels_})
         if (ptr) (void)hipFree(ptr);
     if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
     if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
@@ -508,6 +581,11 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     ka.vote = lc_.vote < 0 ? 0 : std::min(lc_.vote, 64);
     ka.vprops = (const float2*)d_vprops_;
     ka.unit = p.unitinmm;
+    ka.plabels = d_plabels_;
+    ka.plabel_words = plabel_words_;
+    ka.plabel_bits = plabel_bits_;
+    ka.plabel_lg = plabel_bits_ == 2 ? 4u : 3u;
+    ka.per_wg = (count - count / 8) / (uint64_t)std::max(blocks_, 1);
     if (tile_[4] > 0) {
         ka.tx0 = tile_[0]; ka.ty0 = tile_[1]; ka.tz0 = tile_[2]; ka.tg0 = tile_[3];
         ka.tsx = tile_[4]; ka.tsy = tile_[5]; ka.tsz = tile_[6]; ka.tsg = tile_[7];
@@ -518,7 +596,8 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
 }
 
 size_t Session::lds_bytes(uint32_t tile_n) const {
-    return 32 * 16 + (size_t)prob_.mediumnum * 16 + (size_t)tile_n * 4 + (size_t)prob_.ppath_columns() * block_ * 4;
+    return 32 * 16 + (size_t)prob_.mediumnum * 16 + (size_t)plabel_words_ * 4 + (size_t)tile_n * 4 +
+           (size_t)prob_.ppath_columns() * block_ * 4;
 }
 
 // Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
@@ -540,7 +619,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     ka.detnum = 0; ka.ppathlen = 0; ka.count_mode = 1;
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     const bool reflect = p.isreflect != 0;
-    pick(reflect, false, label_bytes_ == 1)(ka, blocks_, block_, lds_bytes(0) - (size_t)p.ppath_columns() * block_ * 4, s);
+    pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0) - (size_t)p.ppath_columns() * block_ * 4, s);
     HIP_OK(hipGetLastError());
     std::vector<float> cnt(fluence_len_);
     HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));
@@ -574,7 +653,7 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    pick(reflect, savedet, label_bytes_ == 1)(ka, blocks_, block_, lds, s);
+    pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -639,7 +718,7 @@ RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    pick_replay(prob_.isreflect != 0, label_bytes_ == 1)(ka, blocks_, block_, lds_bytes(0) - (size_t)prob_.ppath_columns() * block_ * 4, s);
+    pick_replay(prob_.isreflect != 0, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0) - (size_t)prob_.ppath_columns() * block_ * 4, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));

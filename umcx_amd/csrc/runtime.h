@@ -19,6 +19,7 @@ struct LaunchConfig {
     int blocks = 0;          // persistent workgroups (0: fill the GPU)
     int tally_mode = -1;     // -1 tuned, 0 global atomics, 1 LDS-cached tile
     int vote = -1;           // scatter-round threshold in lanes (0: scatter right away, -1: tuned)
+    int label_cache = 0;     // 0: labels in LDS when they fit, -1: labels read from HBM
     bool trace = false;      // keep each photon's final state (parity tests; small runs only)
 };
 
@@ -68,6 +69,7 @@ public:
     int block() const { return block_; }
     int blocks() const { return blocks_; }
     int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
+    int lds_label_bits() const { return (int)plabel_bits_; }  // 0 when labels are read from HBM
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
     float* device_fluence() const { return d_fluence_; }
@@ -100,6 +102,8 @@ private:
     int tile_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool tile_chosen_ = false;
     uint64_t tile_budget_ = 0;
+    uint32_t* d_plabels_ = nullptr;      // packed media ids for LDS staging (null: not staged)
+    uint32_t plabel_words_ = 0, plabel_bits_ = 0;
 
     void* build_args(uint64_t first, uint64_t count, void* out) const;
     size_t lds_bytes(uint32_t tile_n) const;
