@@ -11,7 +11,9 @@ resident inputs (the label volume and media are uploaded once, outside the timed
 Prints one JSON line. ``roofline`` prices the photon kernel's algorithmic FP32 work (per-photon
 op model of the photon loop, counted from src/umcx.cpp:315-428: 34 per step, 9 per voxel crossing,
 62 per scatter, 1 per random draw, 35 per Fresnel test) against the MI355X FP32 peak, using the
-kernel's HIP-event time on its own stream. ``cpu_baseline`` times the reference umcx binary built
+kernel's HIP-event time on its own stream; ``roofline.valu_issue`` gives the bound the kernel
+actually meets, the VALU instruction issue rate (instructions per photon from rocprofv3 counters)
+with the active-lane fraction. ``cpu_baseline`` times the reference umcx binary built
 from source (or, when it is absent, the CPU oracle) on this machine's host cores over a bounded
 sample of the same workload.
 """
@@ -35,6 +37,14 @@ PHOTONS = 100_000_000
 COUNTS = {"S": 346.63, "X": 189.12, "K": 157.29, "R": 475.06, "F": 3.649}
 OPS_PER_PHOTON = 34 * COUNTS["S"] + 9 * COUNTS["X"] + 62 * COUNTS["K"] + COUNTS["R"] + 35 * COUNTS["F"]
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 (vector/matrix) peak
+# VALU issue: a wave64 FP32 instruction occupies a 16-lane SIMD for 4 cycles, so 256 CUs x 4 SIMDs
+# x 2.4 GHz / 4 = 6.14e11 wave-instructions/s (an FMA-chain probe reaches 6.03e11). The photon
+# kernel's instruction count per photon and active-lane fraction on this workload come from
+# rocprofv3 counters (profiles/photon_kernel_hetero_reflective_1e8.md: SQ_INSTS_VALU 4.464e10 and
+# SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU) = 0.373 over 1e7 photons).
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+VALU_INSTR_PER_PHOTON = 4464.0
+ACTIVE_LANE_FRAC = 0.373
 
 CONFIG = {
     "Session": {"ID": "hetero_reflective", "Photons": PHOTONS, "RNGSeed": 1648335518, "DoMismatch": 1,
@@ -296,7 +306,12 @@ def main():
                    "photons_per_step": args.photons, "seed": 1648335518},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5), "traffic": None,
-                     "ops_per_photon": round(OPS_PER_PHOTON, 1), "kernel_ms": round(kavg, 3)},
+                     "ops_per_photon": round(OPS_PER_PHOTON, 1), "kernel_ms": round(kavg, 3),
+                     "valu_issue": {"instr_per_photon": VALU_INSTR_PER_PHOTON,
+                                    "achieved_per_s": VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3),
+                                    "peak_per_s": VALU_ISSUE_PEAK,
+                                    "frac": round(VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3) / VALU_ISSUE_PEAK, 4),
+                                    "active_lane_frac": ACTIVE_LANE_FRAC}},
         "tally_box": sim_box,
         "detected_per_photon": detected / args.photons,
         "absorbed_percent": (tot - esc) / tot * 100.0,
