@@ -99,6 +99,9 @@ class BatchSimulation:
         return ms.value
 
     def fluence(self, j: int) -> np.ndarray:
+        """Raw tallies of job j, shape [t, x, y, z]; empty when the job saves no volume."""
+        if not self.infos[j]["issavevol"]:
+            return np.empty(0, dtype=np.float32)
         nx, ny, nz, nt = self.infos[j]["dims"]
         out = np.empty(nt * nx * ny * nz, dtype=np.float32)
         check(lib().umcx_batch_fluence(self._h, j, out.ctypes.data, out.size))
