@@ -45,6 +45,11 @@ FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 (vector/matrix) peak
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 VALU_INSTR_PER_PHOTON = 4464.0
 ACTIVE_LANE_FRAC = 0.373
+# HBM traffic per photon of the same kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over 1e7
+# photons: 1,206 KiB and 41,950,000 KiB, scaled by 2.0 and 1.0 -- what the two counters read for
+# a 256 MiB device-to-device copy); almost all of it is the memory-side float atomics' 32-byte
+# writes.
+HBM_BYTES_PER_PHOTON = (1206 * 2.0 + 41_950_000 * 1.0) * 1024 / 1e7
 
 CONFIG = {
     "Session": {"ID": "hetero_reflective", "Photons": PHOTONS, "RNGSeed": 1648335518, "DoMismatch": 1,
@@ -305,7 +310,9 @@ def main():
                                "Fresnel on, pencil (29.5,29.5,0)+z, 50 gates x 0.1 ns, 4 disk detectors r=1",
                    "photons_per_step": args.photons, "seed": 1648335518},
         "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
+                     "traffic": round(HBM_BYTES_PER_PHOTON * args.photons),
+                     "traffic_gbps": round(HBM_BYTES_PER_PHOTON * args.photons / (kavg * 1e-3) / 1e9, 1),
                      "ops_per_photon": round(OPS_PER_PHOTON, 1), "kernel_ms": round(kavg, 3),
                      "valu_issue": {"instr_per_photon": VALU_INSTR_PER_PHOTON,
                                     "achieved_per_s": VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3),
