@@ -50,6 +50,10 @@ ACTIVE_LANE_FRAC = 0.373
 # a 256 MiB device-to-device copy); almost all of it is the memory-side float atomics' 32-byte
 # writes.
 HBM_BYTES_PER_PHOTON = (1206 * 2.0 + 41_950_000 * 1.0) * 1024 / 1e7
+# memory-side float atomics per photon (TCC_ATOMIC 1.342e9 over 1e7 photons) against the rate
+# umcx_amd/bin/roofline_probe measures for hashed addresses (profiles/roofline_probe_gfx950.md)
+ATOMICS_PER_PHOTON = 134.2
+ATOMIC_PEAK = 2.11e10
 
 CONFIG = {
     "Session": {"ID": "hetero_reflective", "Photons": PHOTONS, "RNGSeed": 1648335518, "DoMismatch": 1,
@@ -318,7 +322,11 @@ def main():
                                     "achieved_per_s": VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3),
                                     "peak_per_s": VALU_ISSUE_PEAK,
                                     "frac": round(VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3) / VALU_ISSUE_PEAK, 4),
-                                    "active_lane_frac": ACTIVE_LANE_FRAC}},
+                                    "active_lane_frac": ACTIVE_LANE_FRAC},
+                     "atomics": {"per_photon": ATOMICS_PER_PHOTON,
+                                 "achieved_per_s": ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3),
+                                 "peak_per_s": ATOMIC_PEAK,
+                                 "frac": round(ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3) / ATOMIC_PEAK, 4)}},
         "tally_box": sim_box,
         "detected_per_photon": detected / args.photons,
         "absorbed_percent": (tot - esc) / tot * 100.0,

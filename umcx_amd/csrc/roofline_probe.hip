@@ -1,0 +1,94 @@
+// roofline_probe.hip -- measures the two gfx950 ceilings the photon kernels run against.
+//
+//   valu:    wave64 FP32 FMA issue rate (8 independent FMA chains per lane, 16 per unrolled step,
+//            32 waves per SIMD over the run), in wave instructions per second;
+//   atomics: device-scope float atomic adds (no return) to hashed addresses in a 40 MiB buffer
+//            (the size of a time-gated fluence grid), in atomics per second, and the same into
+//            one hot 256-float region (same-address serialisation at the memory side).
+//
+// Prints one JSON line. Every index is taken modulo the buffer size; the launches are bounded.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+
+#define CK(x)                                                                                  \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) {                                                                \
+            fprintf(stderr, "HIP error %s at line %d\n", hipGetErrorString(e_), __LINE__);      \
+            exit(1);                                                                           \
+        }                                                                                      \
+    } while (0)
+
+__global__ void __launch_bounds__(256) fma_chains(float* out, int iters, float a, float b) {
+    float x0 = threadIdx.x, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+    for (int i = 0; i < iters; i++) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            x0 = __builtin_fmaf(x0, a, b); x1 = __builtin_fmaf(x1, a, b);
+            x2 = __builtin_fmaf(x2, a, b); x3 = __builtin_fmaf(x3, a, b);
+            x4 = __builtin_fmaf(x4, a, b); x5 = __builtin_fmaf(x5, a, b);
+            x6 = __builtin_fmaf(x6, a, b); x7 = __builtin_fmaf(x7, a, b);
+        }
+    }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+
+__device__ __forceinline__ uint32_t mix(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+
+__global__ void __launch_bounds__(256) spread_atomics(float* buf, uint32_t n, uint32_t iters) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t k = 0; k < iters; k++) unsafeAtomicAdd(buf + mix(tid * 2654435761u + k) % n, 1.0f);
+}
+
+int main() {
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    auto time_ms = [&](auto launch) {
+        CK(hipEventRecord(e0));
+        launch();
+        CK(hipGetLastError());
+        CK(hipEventRecord(e1));
+        CK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CK(hipEventElapsedTime(&ms, e0, e1));
+        return (double)ms;
+    };
+
+    // VALU: 32768 workgroups x 256 lanes x 2000 x 128 FMAs
+    const int fblocks = 32768, fthreads = 256, fiters = 2000;
+    float* fout;
+    CK(hipMalloc(&fout, (size_t)fblocks * fthreads * 4));
+    auto fma_run = [&] { hipLaunchKernelGGL(fma_chains, dim3(fblocks), dim3(fthreads), 0, 0, fout, fiters, 0.999f, 0.001f); };
+    time_ms(fma_run);
+    const double fms = time_ms(fma_run);
+    const double winstr = (double)fblocks * fthreads / 64 * fiters * 128;
+
+    // atomics: 4096 workgroups x 256 lanes x 256 adds
+    const uint32_t n = 10u << 20, ablocks = 4096, athreads = 256, aiters = 256;
+    float* buf;
+    CK(hipMalloc(&buf, (size_t)n * 4));
+    CK(hipMemset(buf, 0, (size_t)n * 4));
+    auto spread = [&] { hipLaunchKernelGGL(spread_atomics, dim3(ablocks), dim3(athreads), 0, 0, buf, n, aiters); };
+    auto hot = [&] { hipLaunchKernelGGL(spread_atomics, dim3(ablocks), dim3(athreads), 0, 0, buf, 256u, aiters); };
+    time_ms(spread);
+    const double sms = time_ms(spread);
+    const double hms = time_ms(hot);
+    const double natom = (double)ablocks * athreads * aiters;
+
+    hipDeviceProp_t prop;
+    CK(hipGetDeviceProperties(&prop, 0));
+    printf("{\"device\": \"%s\", \"cus\": %d, \"valu_wave_instr_per_s\": %.4e, \"valu_ms\": %.3f, "
+           "\"spread_atomics_per_s\": %.4e, \"hot_atomics_per_s\": %.4e}\n",
+           prop.gcnArchName, prop.multiProcessorCount, winstr / (fms * 1e-3), fms, natom / (sms * 1e-3),
+           natom / (hms * 1e-3));
+    CK(hipFree(fout));
+    CK(hipFree(buf));
+    return 0;
+}
