@@ -105,6 +105,14 @@ int umcx_session_detp_device(const umcx_session* s, uint32_t** ids, float** weig
 int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n,
                         float* grad_mua, float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms);
 
+/* umcx_session_replay for the records of several sources in one launch: record k is launched
+ * from source j = src_index[k], whose position and direction are sources[6j..6j+5] = {pos xyz,
+ * dir xyz} (all device pointers); the source type and its parameters are the session's. No
+ * counterpart in the reference. */
+int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef,
+                                const int32_t* src_index, const float* sources, uint64_t n, float* grad_mua,
+                                float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms);
+
 /* The LDS tally tile in use, {x0, y0, z0, gate0, nx, ny, nz, ngates} (zeros when none).
  * No counterpart in the reference. */
 int umcx_session_tally_box(const umcx_session* s, int32_t box[8]);

@@ -194,3 +194,24 @@ def test_autograd_function_matches_vjp():
     torch.testing.assert_close(mua.grad, ref_mua, rtol=1e-4, atol=1e-6 * float(ref_mua.abs().max()))
     torch.testing.assert_close(mus.grad, ref_mus, rtol=1e-4, atol=1e-6 * float(ref_mus.abs().max()))
     m.close()
+
+
+def test_one_launch_replay_equals_per_source_replays():
+    """umcx_session_replay_sources (every source's records in one launch) adds the same
+    gradients as one umcx_session_replay per source (float summation order aside)."""
+    import torch
+    cfg, m = c4_model(nsrc_side=2, nphoton=100000)
+    say("one-launch replay vs per-source replays")
+    m.forward()
+    Jall = m.jacobian()
+    S, D = len(m.sources), m.ndet
+    J2 = torch.zeros((2, S * D, m.nvox), dtype=torch.float32, device="cuda")
+    for s in range(S):
+        ids, det, w = m.records[s]
+        Isd = m.I[s, det]
+        coef = np.where(Isd > 0, w / np.where(Isd > 0, Isd, 1.0), 0.0)
+        m._replay(s, coef, (s * D + det).astype(np.int32), J2[0], J2[1])
+    J2 = J2.permute(1, 2, 0)
+    a, b = Jall.cpu().numpy(), J2.cpu().numpy()
+    assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    m.close()

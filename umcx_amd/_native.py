@@ -58,6 +58,9 @@ SIGNATURES = {
                                            C.POINTER(C.c_void_p)]),
     "umcx_session_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                       C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_double)]),
+    "umcx_session_replay_sources": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
+                                              C.c_void_p, C.POINTER(C.c_double)]),
     "umcx_session_free": (None, [C.c_void_p]),
     "umcx_batch_create": (C.c_int, [C.POINTER(C.c_char_p), C.c_int32, C.POINTER(UmcxLaunch), C.POINTER(C.c_void_p)]),
     "umcx_batch_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),

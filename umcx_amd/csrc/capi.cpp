@@ -271,6 +271,18 @@ int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* sli
     });
 }
 
+int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef,
+                                const int32_t* src_index, const float* sources, uint64_t n, float* grad_mua,
+                                float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms) {
+    return guarded([&]() -> int {
+        if (!src_index || !sources) throw std::runtime_error("umcx_session_replay_sources: null source arrays");
+        umcx::RunStats st = s->s->replay(ids, slices, coef, n, grad_mua, grad_mus, slice_stride, final_w, src_index,
+                                         sources);
+        if (kernel_ms) *kernel_ms = st.kernel_ms;
+        return 0;
+    });
+}
+
 int umcx_session_tally_box(const umcx_session* s, int32_t box[8]) {
     return guarded([&]() -> int {
         for (int i = 0; i < 8; i++) box[i] = s->s->tile()[i];

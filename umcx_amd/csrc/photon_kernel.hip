@@ -82,6 +82,9 @@ struct KernelArgs {
     // photon slots: workgroup w first takes [w*per_wg, (w+1)*per_wg) through an LDS counter,
     // then slots from blocks*per_wg on through the device cursor
     uint64_t per_wg;
+    // replay of several sources at once: source of each record and {pos xyz, dir xyz} per source
+    const int32_t* rsrc;
+    const float* rsrcs;
 };
 
 template <bool LDSLAB, typename LabelT>
@@ -134,7 +137,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     if (tid == 0) s_taken = 0u;
     const Tabs tabs{slog, sexp};
     const Grid grid{a.nx, a.ny, a.nz, a.nyz};
-    const Source src{a.srcpos, a.srcdir, a.srcp1, a.srcp2, a.srctype};
+    Source src{a.srcpos, a.srcdir, a.srcp1, a.srcp2, a.srctype};
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
     if (LDSLAB)
         for (uint32_t i = tid; i < a.plabel_words; i += bd) slab[i] = a.plabels[i];
@@ -187,6 +190,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 rcoef = a.rcoef[k];
                 rbase = (uint64_t)a.rslice[k] * a.gstride;
                 seg_ns = 0.f;
+                if (a.rsrc) {
+                    const float* q = a.rsrcs + 6 * (uint64_t)a.rsrc[k];
+                    src.srcpos.x = q[0]; src.srcpos.y = q[1]; src.srcpos.z = q[2];
+                    src.srcdir.x = q[3]; src.srcdir.y = q[4]; src.srcdir.z = q[5];
+                }
             }
             const uint32_t i32 = (uint32_t)pid;
             rng_seed(r, a.seeds[0] ^ i32, a.seeds[1] | i32, a.seeds[2] ^ i32, a.seeds[3] | i32);   // :761
@@ -701,7 +709,8 @@ void Session::set_source(const float pos[3], const float dir[3]) {
 }
 
 RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n, float* gmua,
-                         float* gmus, uint64_t stride, float* final_w) {
+                         float* gmus, uint64_t stride, float* final_w, const int32_t* src_index,
+                         const float* sources) {
     HIP_OK(hipSetDevice(lc_.device));
     hipStream_t s = (hipStream_t)stream_;
     RunStats st;
@@ -714,6 +723,8 @@ RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     ka.rids = ids; ka.rslice = slices; ka.rcoef = coef;
     ka.gmua = gmua; ka.gmus = gmus; ka.gstride = stride; ka.rw = final_w;
+    ka.rsrc = sources ? src_index : nullptr;
+    ka.rsrcs = sources;
     ka.vote = 0;
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());

@@ -57,9 +57,12 @@ public:
     void set_source(const float pos[3], const float dir[3]);
     // re-trace photons ids[0..n) (device arrays) adding coef[k] x perturbation-MC derivatives of
     // the photon's weight (mua) and path density (mus) into slice slices[k] of gmua / gmus
-    // (device, slice stride `stride` floats, voxels in label order); final weights to final_w
+    // (device, slice stride `stride` floats, voxels in label order); final weights to final_w.
+    // With `sources` (device, {pos xyz, dir xyz} per source), record k is launched from source
+    // src_index[k] instead of the session's source.
     RunStats replay(const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n, float* gmua,
-                    float* gmus, uint64_t stride, float* final_w);
+                    float* gmus, uint64_t stride, float* final_w, const int32_t* src_index = nullptr,
+                    const float* sources = nullptr);
     uint32_t* device_detid() const { return d_detid_; }
     float* device_detw() const { return d_detw_; }
     float* device_detp() const { return d_detp_; }

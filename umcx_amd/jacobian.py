@@ -10,8 +10,9 @@ perturbation Monte Carlo on the detected photons' own paths:
 with L_pv the path length (mm) and n_pv the number of scatterings of photon p in voxel v. Photon
 i is a pure function of i (src/umcx.cpp:761), so the forward keeps only (photon index, detector,
 final weight) per detection and the backward *replays* exactly those photons with a hand-written
-kernel (``umcx_session_replay``) that adds coef_p x the two terms along the path: coef = w/I for
-the Jacobian of log I, coef = w g_sd / I_sd for a vector-Jacobian product. Under common random
+kernel (``umcx_session_replay_sources``: the records of every source in one launch) that adds
+coef_p x the two terms along the path: coef = w/I for the Jacobian of log I, coef = w g_sd / I_sd
+for a vector-Jacobian product. Under common random
 numbers a mua perturbation leaves every trajectory unchanged, so central differences of this
 estimator match the replay gradient up to O(h^2).
 
@@ -119,6 +120,29 @@ class MultiSourceModel:
                                     C.c_void_p(final_w.data_ptr()) if final_w is not None else None, C.byref(ms)))
         return ms.value
 
+    def _replay_all(self, coefs, slices, gmua, gmus):
+        """Replay the records of every source in one launch (umcx_session_replay_sources)."""
+        torch = _torch()
+        dev = f"cuda:{self.device}"
+        ids = np.concatenate([r[0] for r in self.records]) if self.records else np.zeros(0, np.int32)
+        n = len(ids)
+        if n == 0:
+            return 0.0
+        src = np.concatenate([np.full(len(r[0]), s, np.int32) for s, r in enumerate(self.records)])
+        table = np.array([list(p) + list(d) for p, d in self.sources], dtype=np.float32).reshape(-1)
+        t_ids = torch.from_numpy(ids.astype(np.int32)).to(dev)
+        t_sl = torch.from_numpy(np.ascontiguousarray(np.concatenate(slices), dtype=np.int32)).to(dev)
+        t_cf = torch.from_numpy(np.ascontiguousarray(np.concatenate(coefs), dtype=np.float32)).to(dev)
+        t_src = torch.from_numpy(src).to(dev)
+        t_tab = torch.from_numpy(table).to(dev)
+        torch.cuda.synchronize()
+        ms = C.c_double(0.0)
+        check(lib().umcx_session_replay_sources(
+            self.sim._h, C.c_void_p(t_ids.data_ptr()), C.c_void_p(t_sl.data_ptr()), C.c_void_p(t_cf.data_ptr()),
+            C.c_void_p(t_src.data_ptr()), C.c_void_p(t_tab.data_ptr()), n, C.c_void_p(gmua.data_ptr()),
+            C.c_void_p(gmus.data_ptr()), self.nvox, None, C.byref(ms)))
+        return ms.value
+
     def vjp(self, grad_logI):
         """sum_sd grad[s,d] * d log I_sd / d(mua, mus): two (nx, ny, nz) float32 cuda tensors."""
         torch = _torch()
@@ -127,12 +151,13 @@ class MultiSourceModel:
         dev = f"cuda:{self.device}"
         gmua = torch.zeros(self.nvox, dtype=torch.float32, device=dev)
         gmus = torch.zeros(self.nvox, dtype=torch.float32, device=dev)
-        self.replay_ms = 0.0
+        coefs, slices = [], []
         for s in range(len(self.sources)):
             ids, det, w = self.records[s]
             Isd = self.I[s, det]
-            coef = np.where(Isd > 0, w * g[s, det] / np.where(Isd > 0, Isd, 1.0), 0.0)
-            self.replay_ms += self._replay(s, coef, np.zeros(len(ids), np.int32), gmua, gmus)
+            coefs.append(np.where(Isd > 0, w * g[s, det] / np.where(Isd > 0, Isd, 1.0), 0.0))
+            slices.append(np.zeros(len(ids), np.int32))
+        self.replay_ms = self._replay_all(coefs, slices, gmua, gmus)
         return gmua.view(self.shape), gmus.view(self.shape)
 
     def jacobian(self):
@@ -141,12 +166,13 @@ class MultiSourceModel:
         dev = f"cuda:{self.device}"
         S, D = len(self.sources), self.ndet
         J = torch.zeros((2, S * D, self.nvox), dtype=torch.float32, device=dev)
-        self.replay_ms = 0.0
+        coefs, slices = [], []
         for s in range(S):
             ids, det, w = self.records[s]
             Isd = self.I[s, det]
-            coef = np.where(Isd > 0, w / np.where(Isd > 0, Isd, 1.0), 0.0)
-            self.replay_ms += self._replay(s, coef, (s * D + det).astype(np.int32), J[0], J[1])
+            coefs.append(np.where(Isd > 0, w / np.where(Isd > 0, Isd, 1.0), 0.0))
+            slices.append((s * D + det).astype(np.int32))
+        self.replay_ms = self._replay_all(coefs, slices, J[0], J[1])
         return J.permute(1, 2, 0)
 
     def replay_weights(self, s: int):
