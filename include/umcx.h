@@ -105,6 +105,11 @@ int umcx_session_detp_device(const umcx_session* s, uint32_t** ids, float** weig
 int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n,
                         float* grad_mua, float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms);
 
+/* Several sources in one run: sources = {pos xyz, dir xyz} per source (host), photon slot p of
+ * umcx_session_run is photon p mod photons_per_source of source p / photons_per_source, seeded
+ * and traced exactly as in that source's own run; detection records carry the slot. NULL or
+ * nsrc 0 returns to the single source. No counterpart in the reference. */
+int umcx_session_set_sources(umcx_session* s, const float* sources, int32_t nsrc, uint64_t photons_per_source);
 /* umcx_session_replay for the records of several sources in one launch: record k is launched
  * from source j = src_index[k], whose position and direction are sources[6j..6j+5] = {pos xyz,
  * dir xyz} (all device pointers); the source type and its parameters are the session's. No

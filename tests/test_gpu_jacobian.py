@@ -215,3 +215,31 @@ def test_one_launch_replay_equals_per_source_replays():
     a, b = Jall.cpu().numpy(), J2.cpu().numpy()
     assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
     m.close()
+
+
+def test_several_sources_in_one_run_equal_separate_runs():
+    """umcx_session_set_sources: photon p % N of source p // N is the photon a run of that source
+    alone traces (same rows bitwise, same final weights)."""
+    import ctypes as C
+    from umcx_amd._native import check, lib
+    g = O.golden("c4src")
+    cfg = copy.deepcopy(g["cfg"])
+    cfg["Session"]["Photons"] = 30000
+    srcs = J.grid_sources(64, 64, 2, 16.5)
+    N = 30000
+    say("several sources in one run")
+    table = np.array([list(p) + list(d) for p, d in srcs], dtype=np.float32).reshape(-1)
+    with U.Simulation(cfg, tally_mode=0) as sim:
+        check(lib().umcx_session_set_sources(sim._h, table.ctypes.data, len(srcs), N))
+        sim.run(0, len(srcs) * N)
+        rows, slots, w = sim.detp()
+        check(lib().umcx_session_set_sources(sim._h, None, 0, 0))
+        for s, (pos, dirv) in enumerate(srcs):
+            check(lib().umcx_session_set_source(sim._h, (C.c_float * 3)(*pos), (C.c_float * 3)(*dirv)))
+            sim.reset()
+            sim.run(0, N)
+            r1, i1, w1 = sim.detp()
+            m = (slots // N) == s
+            got = np.column_stack([rows[m], (slots[m] - s * N).astype(np.float32), w[m]])
+            ref = np.column_stack([r1, i1.astype(np.float32), w1])
+            assert np.array_equal(srt(got).view(np.uint32), srt(ref).view(np.uint32)), s

@@ -58,6 +58,7 @@ SIGNATURES = {
                                            C.POINTER(C.c_void_p)]),
     "umcx_session_replay": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                                       C.c_void_p, C.c_uint64, C.c_void_p, C.POINTER(C.c_double)]),
+    "umcx_session_set_sources": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint64]),
     "umcx_session_replay_sources": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64,
                                               C.c_void_p, C.POINTER(C.c_double)]),

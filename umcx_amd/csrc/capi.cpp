@@ -271,6 +271,10 @@ int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* sli
     });
 }
 
+int umcx_session_set_sources(umcx_session* s, const float* sources, int32_t nsrc, uint64_t photons_per_source) {
+    return guarded([&]() -> int { s->s->set_sources(sources, nsrc, photons_per_source); return 0; });
+}
+
 int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef,
                                 const int32_t* src_index, const float* sources, uint64_t n, float* grad_mua,
                                 float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms) {

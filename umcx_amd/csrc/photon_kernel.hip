@@ -82,9 +82,11 @@ struct KernelArgs {
     // photon slots: workgroup w first takes [w*per_wg, (w+1)*per_wg) through an LDS counter,
     // then slots from blocks*per_wg on through the device cursor
     uint64_t per_wg;
-    // replay of several sources at once: source of each record and {pos xyz, dir xyz} per source
+    // several sources in one launch, {pos xyz, dir xyz} per source: in replay the source of each
+    // record is rsrc[k]; otherwise photon slot p is photon p mod msrc_n of source p / msrc_n
     const int32_t* rsrc;
     const float* rsrcs;
+    uint64_t msrc_n;
 };
 
 template <bool LDSLAB, typename LabelT>
@@ -196,7 +198,15 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                     src.srcdir.x = q[3]; src.srcdir.y = q[4]; src.srcdir.z = q[5];
                 }
             }
-            const uint32_t i32 = (uint32_t)pid;
+            uint64_t ip = pid;                       // photon index within its source
+            if (!REPLAY && a.msrc_n) {
+                const uint64_t si = pid / a.msrc_n;
+                ip = pid - si * a.msrc_n;
+                const float* q = a.rsrcs + 6 * si;
+                src.srcpos.x = q[0]; src.srcpos.y = q[1]; src.srcpos.z = q[2];
+                src.srcdir.x = q[3]; src.srcdir.y = q[4]; src.srcdir.z = q[5];
+            }
+            const uint32_t i32 = (uint32_t)ip;
             rng_seed(r, a.seeds[0] ^ i32, a.seeds[1] | i32, a.seeds[2] ^ i32, a.seeds[3] | i32);   // :761
             w0 = launch(p, r, src);
             ph_dep = 0.0;
@@ -546,7 +556,7 @@ Session::~Session() {
     (void)hipSetDevice(lc_.device);
     for (void* ptr : {(void*)d_labels_, (void*)d_media_, (void*)d_detpos_, (void*)d_fluence_, (void*)d_detp_,
                       (void*)d_detid_, (void*)d_detw_, (void*)d_counters_, (void*)d_cursor_, (void*)d_energy_,
-                      (void*)d_trace_, d_vprops_, (void*)d_plabels_})
+                      (void*)d_trace_, d_vprops_, (void*)d_plabels_, (void*)d_msrc_})
         if (ptr) (void)hipFree(ptr);
     if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
     if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
@@ -594,6 +604,8 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     ka.plabel_bits = plabel_bits_;
     ka.plabel_lg = plabel_bits_ == 2 ? 4u : 3u;
     ka.per_wg = (count - count / 8) / (uint64_t)std::max(blocks_, 1);
+    ka.msrc_n = msrc_n_;
+    ka.rsrcs = msrc_n_ ? d_msrc_ : nullptr;
     if (tile_[4] > 0) {
         ka.tx0 = tile_[0]; ka.ty0 = tile_[1]; ka.tz0 = tile_[2]; ka.tg0 = tile_[3];
         ka.tsx = tile_[4]; ka.tsy = tile_[5]; ka.tsz = tile_[6]; ka.tsg = tile_[7];
@@ -699,6 +711,21 @@ void Session::set_voxel_props(const float* mua, const float* mus, bool on_device
     for (int& v : tile_) v = 0;
 }
 
+void Session::set_sources(const float* table, int nsrc, uint64_t per_source) {
+    HIP_OK(hipSetDevice(lc_.device));
+    if (d_msrc_) HIP_OK(hipFree(d_msrc_));
+    d_msrc_ = nullptr;
+    msrc_n_ = 0;
+    if (!table || nsrc <= 0 || per_source == 0) return;
+    if ((uint64_t)nsrc * per_source > 0xFFFFFFFFull)
+        throw std::runtime_error("several sources in one run: at most 2^32 photons in total");
+    d_msrc_ = dalloc<float>((size_t)nsrc * 6);
+    HIP_OK(hipMemcpy(d_msrc_, table, (size_t)nsrc * 6 * 4, hipMemcpyHostToDevice));
+    msrc_n_ = per_source;
+    tile_chosen_ = false;
+    for (int& v : tile_) v = 0;
+}
+
 void Session::set_source(const float pos[3], const float dir[3]) {
     for (int i = 0; i < 3; i++) {
         prob_.srcpos[i] = pos[i];
@@ -725,6 +752,7 @@ RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float
     ka.gmua = gmua; ka.gmus = gmus; ka.gstride = stride; ka.rw = final_w;
     ka.rsrc = sources ? src_index : nullptr;
     ka.rsrcs = sources;
+    ka.msrc_n = 0;
     ka.vote = 0;
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());

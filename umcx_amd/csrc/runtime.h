@@ -55,6 +55,9 @@ public:
     void set_voxel_props(const float* mua, const float* mus, bool on_device);
     // move the source (position, direction) for the next runs
     void set_source(const float pos[3], const float dir[3]);
+    // several sources in one run (host table {pos xyz, dir xyz} per source; null: off): photon
+    // slot p runs photon p mod per_source of source p / per_source, seeded as in its own run
+    void set_sources(const float* table, int nsrc, uint64_t per_source);
     // re-trace photons ids[0..n) (device arrays) adding coef[k] x perturbation-MC derivatives of
     // the photon's weight (mua) and path density (mus) into slice slices[k] of gmua / gmus
     // (device, slice stride `stride` floats, voxels in label order); final weights to final_w.
@@ -105,6 +108,8 @@ private:
     int tile_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool tile_chosen_ = false;
     uint64_t tile_budget_ = 0;
+    float* d_msrc_ = nullptr;            // multi-source table {pos xyz, dir xyz} per source
+    uint64_t msrc_n_ = 0;                // photons per source (0: one source)
     uint32_t* d_plabels_ = nullptr;      // packed media ids for LDS staging (null: not staged)
     uint32_t plabel_words_ = 0, plabel_bits_ = 0;
 

@@ -48,7 +48,7 @@ class MultiSourceModel:
         S["DoPartialPath"] = 1
         S["SaveDetFlag"] = 1
         S["Photons"] = int(nphoton)
-        S["MaxDetPhoton"] = int(max_records or max(1_000_000, nphoton // 4))
+        S["MaxDetPhoton"] = int(max_records or len(sources) * max(1_000_000, nphoton // 4))
         self.cfg = cfg
         self.sources = [(list(map(float, p)), list(map(float, d))) for p, d in sources]
         self.nphoton = int(nphoton)
@@ -83,17 +83,28 @@ class MultiSourceModel:
         """Simulate every source; returns log I as a (sources, detectors) float64 array."""
         self._set_props(mua, mus)
         self.records = []
-        I = np.zeros((len(self.sources), self.ndet), dtype=np.float64)
+        S, N = len(self.sources), self.nphoton
+        I = np.zeros((S, self.ndet), dtype=np.float64)
         L = lib()
-        self.forward_ms = 0.0
-        for s, (pos, dirv) in enumerate(self.sources):
-            check(L.umcx_session_set_source(self.sim._h, (C.c_float * 3)(*pos), (C.c_float * 3)(*dirv)))
-            self.sim.reset()
-            self.forward_ms += self.sim.run(0, self.nphoton)
-            rows, ids, w = self.sim.detp()
-            det = rows[:, 0].astype(np.int64) - 1
-            np.add.at(I[s], det, w.astype(np.float64))
-            self.records.append((ids.astype(np.int32), det, w))
+        # every source in one launch: photon slot p is photon p % N of source p // N
+        table = np.array([list(p) + list(d) for p, d in self.sources], dtype=np.float32).reshape(-1)
+        check(L.umcx_session_set_sources(self.sim._h, table.ctypes.data, S, N))
+        self.sim.reset()
+        self.forward_ms = self.sim.run(0, S * N)
+        check(L.umcx_session_set_sources(self.sim._h, None, 0, 0))
+        detected, saved = self.sim.detected()
+        if saved < detected:
+            raise RuntimeError(f"{detected - saved} detected photons did not fit MaxDetPhoton; raise max_records")
+        rows, slots, w = self.sim.detp()
+        det = rows[:, 0].astype(np.int64) - 1
+        src = slots.astype(np.int64) // N
+        order = np.argsort(src, kind="stable")
+        src, slots, det, w = src[order], slots[order], det[order], w[order]
+        bounds = np.searchsorted(src, np.arange(S + 1))
+        for s in range(S):
+            a, b = bounds[s], bounds[s + 1]
+            np.add.at(I[s], det[a:b], w[a:b].astype(np.float64))
+            self.records.append(((slots[a:b] - s * N).astype(np.int32), det[a:b], w[a:b]))
         self.I = I
         with np.errstate(divide="ignore"):
             return np.log(I)
