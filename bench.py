@@ -159,6 +159,7 @@ def batched_leg(sync) -> dict:
     log("bench: plain loop over the same jobs")
     res, loop_kernel_ms, loop_wall_ms = B.plain_loop(jobs)
     roi_p = np.array([B.roi_sum(f, c) for (f, _), c in zip(res, jobs)])
+    matched = matched_noise_batch(jobs, chosen, sync)
     return {
         "workload": "256 uneven jobs, seed 7: edges U[32,128], 2-6 box labels, mua U(0.001,0.05), mus U(0.5,20), "
                     "g U(0.7,0.95), n U(1.33,1.45), Fresnel on, pencil at the z=0 face centre, photons log-U[1e5,1e7], "
@@ -174,6 +175,61 @@ def batched_leg(sync) -> dict:
         "reported_quantity": "per-job fluence summed over the 5x5x5 voxels under the source",
         "noise_rel_std_rms": float(np.sqrt(np.mean(rel ** 2))), "noise_rel_std_max_est": float(rel.max()),
         "batch_vs_plain_max_rel_diff": float(np.max(np.abs(roi_b - roi_p) / roi_p)),
+        "at_1pct_noise": matched,
+    }
+
+
+def matched_noise_batch(jobs, chosen, sync, target: float = 0.01, reps: int = 8, margin: float = 3.0) -> dict:
+    """The same comparison at the photon count each job needs for a 1 % relative standard
+    deviation of its reported quantity. Per-job noise comes from `reps` independent replicas of
+    N/reps photons; the count is margin x N (s/target)^2, at least 2000 photons and at most N;
+    the noise reached is checked with a second batch at changed seeds."""
+    import copy
+
+    import numpy as np
+
+    from umcx_amd import batch as B
+    shape = dict(block=chosen["block"], blocks=chosen["blocks"], tally_mode=1 if chosen["tile_cells"] else 0)
+    rep_jobs = []
+    for c in jobs:
+        for r in range(reps):
+            c2 = copy.deepcopy(c)
+            c2["Session"]["Photons"] = max(1, int(c["Session"]["Photons"]) // reps)
+            c2["Session"]["RNGSeed"] = int(c["Session"]["RNGSeed"]) + 1000 * (r + 1)
+            rep_jobs.append(c2)
+    with B.BatchSimulation(rep_jobs, **shape) as bs:
+        bs.run()
+        roi = np.array([B.roi_sum(bs.fluence(j), rep_jobs[j]) for j in range(len(rep_jobs))]).reshape(len(jobs), reps)
+    rel_full = roi.std(axis=1, ddof=1) / np.sqrt(reps) / roi.mean(axis=1)      # at the full count N
+    small = copy.deepcopy(jobs)
+    for c, s_ in zip(small, rel_full):
+        n = int(c["Session"]["Photons"])
+        c["Session"]["Photons"] = int(min(n, max(2000, np.ceil(margin * n * (s_ / target) ** 2))))
+    photons = sum(int(c["Session"]["Photons"]) for c in small)
+    with B.BatchSimulation(small, **shape) as bs:
+        bs.run()
+        sync()
+        t0 = time.perf_counter()
+        batch_ms = bs.run()
+        batch_wall = (time.perf_counter() - t0) * 1e3
+        roi_a = np.array([B.roi_sum(bs.fluence(j), small[j]) for j in range(len(small))])
+    small2 = copy.deepcopy(small)
+    for c in small2:
+        c["Session"]["RNGSeed"] = int(c["Session"]["RNGSeed"]) + 1
+    with B.BatchSimulation(small2, **shape) as bs:
+        bs.run()
+        roi_b = np.array([B.roi_sum(bs.fluence(j), small2[j]) for j in range(len(small2))])
+    rel = np.abs(roi_a - roi_b) / np.sqrt(2.0) / (0.5 * (roi_a + roi_b))
+    log(f"bench: plain loop at the 1% noise counts ({photons:.3e} photons)")
+    _, loop_kernel_ms, loop_wall_ms = B.plain_loop(small, keep=False)
+    return {
+        "photons": photons, "photons_min": min(int(c["Session"]["Photons"]) for c in small),
+        "photons_max": max(int(c["Session"]["Photons"]) for c in small),
+        "batch_ms": round(batch_ms, 2), "batch_wall_ms": round(batch_wall, 2),
+        "plain_loop_kernel_ms": round(loop_kernel_ms, 2), "plain_loop_wall_ms": round(loop_wall_ms, 2),
+        "speedup_vs_plain_kernel": round(loop_kernel_ms / batch_ms, 3),
+        "speedup_vs_plain_wall": round(loop_wall_ms / batch_wall, 3),
+        "noise_rel_std_rms": float(np.sqrt(np.mean(rel ** 2))), "noise_rel_std_max_est": float(rel.max()),
     }
 
 
