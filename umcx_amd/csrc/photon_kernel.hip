@@ -36,6 +36,7 @@ namespace umcx {
 
 constexpr uint64_t kLabelCacheBytes = 96 * 1024;   // largest packed label volume staged in LDS
 constexpr uint64_t kMaxTileCells = 32768;          // largest LDS tally tile
+constexpr int kRegPaths = 4;                       // partial paths kept in registers up to this many media
 
 namespace dev {
 
@@ -144,7 +145,9 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     if (LDSLAB)
         for (uint32_t i = tid; i < a.plabel_words; i += bd) slab[i] = a.plabels[i];
     for (int i = tid; i < (int)a.tile_n; i += bd) stile[i] = 0.f;
-    if (SAVEDET)
+    // partial path lengths per medium: in registers for up to kRegPaths media, else in LDS
+    const bool regpp = a.ppathlen <= kRegPaths;
+    if (SAVEDET && !regpp)
         for (int i = tid; i < a.ppathlen * bd; i += bd) sppath[i] = 0.f;
     __syncthreads();
     const auto labs = LabelPick<LDSLAB, LabelT>::make(a, slab);
@@ -161,6 +164,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     float w0 = 0.f;
     double e_esc = 0.0, e_tot = 0.0, e_dep = 0.0, ph_dep = 0.0;
     float rcoef = 0.f, seg_ns = 0.f;
+    float pp0 = 0.f, pp1 = 0.f, pp2 = 0.f, pp3 = 0.f;
     uint64_t rbase = 0, rk = 0;
     const bool savevol = !REPLAY && a.issavevol != 0;
     const bool energy_out = a.outputtype == kEnergy;
@@ -210,8 +214,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             rng_seed(r, a.seeds[0] ^ i32, a.seeds[1] | i32, a.seeds[2] ^ i32, a.seeds[3] | i32);   // :761
             w0 = launch(p, r, src);
             ph_dep = 0.0;
-            if (SAVEDET)
-                for (int m = 0; m < a.ppathlen; m++) sppath[m * bd + tid] = 0.f;
+            if (SAVEDET) {
+                pp0 = pp1 = pp2 = pp3 = 0.f;
+                if (!regpp)
+                    for (int m = 0; m < a.ppathlen; m++) sppath[m * bd + tid] = 0.f;
+            }
             p.lastvox = vox_index(grid, p.ix, p.iy, p.iz);                                           // :262
             bool entered = p.lastvox >= 0 || skip_to_box(p, grid);                                   // :264
             if (entered && p.lastvox < 0) entered = false;   // entry exactly on a far face
@@ -258,7 +265,16 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                     p.lz = 0.f;
                     seg_ns = 0.f;
                 }
-                if (SAVEDET && a.ppathlen && m > 0) sppath[(m - 1) * bd + tid] += p.lz;
+                if (SAVEDET && a.ppathlen && m > 0) {
+                    if (regpp) {
+                        pp0 = m == 1 ? pp0 + p.lz : pp0;
+                        pp1 = m == 2 ? pp1 + p.lz : pp1;
+                        pp2 = m == 3 ? pp2 + p.lz : pp2;
+                        pp3 = m == 4 ? pp3 + p.lz : pp3;
+                    } else {
+                        sppath[(m - 1) * bd + tid] += p.lz;
+                    }
+                }
                 if (savevol) {
                     int tshift = (int)fminf((float)(a.maxgate - 1), (float)(int)floorf((p.ly - a.tstart) * a.rtstep));
                     float dw = p.lw - p.pw;
@@ -322,7 +338,12 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                         float* row = a.detp + (uint64_t)slot * a.detlen;
                         int c = 0;
                         if (a.savedetflag & kDetID) row[c++] = (float)(d + 1);
-                        for (int m = 0; m < a.ppathlen; m++) row[c++] = sppath[m * bd + tid];
+                        if (regpp) {
+                            const float pp[kRegPaths] = {pp0, pp1, pp2, pp3};
+                            for (int m = 0; m < a.ppathlen; m++) row[c++] = pp[m];
+                        } else {
+                            for (int m = 0; m < a.ppathlen; m++) row[c++] = sppath[m * bd + tid];
+                        }
                         if (a.savedetflag & kExitPos) { row[c++] = p.px; row[c++] = p.py; row[c++] = p.pz; }
                         if (a.savedetflag & kExitDir) { row[c++] = p.vx; row[c++] = p.vy; row[c++] = p.vz; }
                         a.detid[slot] = (uint32_t)pid;
@@ -615,9 +636,9 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     return out;
 }
 
-size_t Session::lds_bytes(uint32_t tile_n) const {
-    return 32 * 16 + (size_t)prob_.mediumnum * 16 + (size_t)plabel_words_ * 4 + (size_t)tile_n * 4 +
-           (size_t)prob_.ppath_columns() * block_ * 4;
+size_t Session::lds_bytes(uint32_t tile_n, bool paths_needed) const {
+    const size_t paths = paths_needed && prob_.ppath_columns() > kRegPaths ? (size_t)prob_.ppath_columns() * block_ * 4 : 0;
+    return 32 * 16 + (size_t)prob_.mediumnum * 16 + (size_t)plabel_words_ * 4 + (size_t)tile_n * 4 + paths;
 }
 
 // Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
@@ -639,7 +660,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     ka.detnum = 0; ka.ppathlen = 0; ka.count_mode = 1;
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     const bool reflect = p.isreflect != 0;
-    pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0) - (size_t)p.ppath_columns() * block_ * 4, s);
+    pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
     std::vector<float> cnt(fluence_len_);
     HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));
@@ -757,7 +778,7 @@ RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    pick_replay(prob_.isreflect != 0, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0) - (size_t)prob_.ppath_columns() * block_ * 4, s);
+    pick_replay(prob_.isreflect != 0, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));

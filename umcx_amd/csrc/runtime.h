@@ -114,7 +114,7 @@ private:
     uint32_t plabel_words_ = 0, plabel_bits_ = 0;
 
     void* build_args(uint64_t first, uint64_t count, void* out) const;
-    size_t lds_bytes(uint32_t tile_n) const;
+    size_t lds_bytes(uint32_t tile_n, bool paths_needed = true) const;   // dynamic LDS of a launch
     void choose_tile(uint64_t first, uint64_t count);
 
 public:
