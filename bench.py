@@ -309,6 +309,44 @@ def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3) -> dict:
     }
 
 
+SKIN_CONFIG = {
+    "Session": {"ID": "skinvessel200", "Photons": 100_000_000, "RNGSeed": 1648335518, "DoMismatch": 1,
+                "DoSaveVolume": 1, "DoPartialPath": 0, "OutputType": "e"},
+    "Forward": {"T0": 0, "T1": 5e-9, "Dt": 5e-9},
+    "Domain": {"Dim": [200, 200, 200], "LengthUnit": 0.005,
+               "Media": [{"mua": 0, "mus": 0, "g": 1, "n": 1}, {"mua": 1e-5, "mus": 0, "g": 1, "n": 1},
+                         {"mua": 1.657, "mus": 37.59, "g": 0.9, "n": 1.37},
+                         {"mua": 0.0458, "mus": 35.65, "g": 0.9, "n": 1.37},
+                         {"mua": 23.05, "mus": 9.398, "g": 0.9, "n": 1.37}]},
+    "Optode": {"Source": {"Type": "disk", "Pos": [100, 100, 0], "Dir": [0, 0, 1], "Param1": [60, 0, 0, 0]}},
+    "Shapes": [{"Grid": {"Tag": 3, "Size": [200, 200, 200]}}, {"ZLayers": [[0, 20, 1], [20, 32, 2]]},
+               {"Cylinder": {"Tag": 4, "C0": [0, 100, 100], "C1": [200, 100, 100], "R": 20}}]}
+
+
+def other_configs_leg(sync) -> dict:
+    """Photons/ms of the two other single-GPU configurations: the small homogeneous cube (60^3,
+    reflection off, 1e6 photons, no detectors) and the layered skin model with a vessel
+    (200^3 voxels of 5 um, Fresnel on, disk source r=60 voxels, 1e8 photons, energy output)."""
+    import umcx_amd as U
+    out = {}
+    cube = U.config_from_argv(["--bench", "cube60", "-d", "0", "-n", "1000000"])
+    for name, cfg, steps in (("cube60_1e6", cube, 5), ("skinvessel_200cube_1e8", SKIN_CONFIG, 1)):
+        with U.Simulation(cfg) as sim:
+            n = int(float(cfg["Session"]["Photons"]))
+            sim.run()
+            ms = []
+            for _ in range(steps):
+                sim.reset()
+                ms.append(sim.run())
+            kms = sum(ms) / len(ms)
+            esc, tot, dep = sim.energies()
+            out[name] = {"photons": n, "kernel_ms": round(kms, 3), "photons_per_ms": round(n / kms, 1),
+                         "lds_label_bits": sim.info["lds_label_bits"], "tally_box": sim.tally_box(),
+                         "absorbed_percent": (tot - esc) / tot * 100.0}
+        log(f"bench: {name}: {out[name]['photons_per_ms']} photons/ms")
+    return out
+
+
 def O_domain():
     import umcx_amd as U
     return U.domain(CONFIG)
@@ -323,6 +361,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-jacobian", action="store_true", help="skip the Jacobian-vs-plain leg")
+    ap.add_argument("--no-others", action="store_true", help="skip the other configurations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -387,6 +426,8 @@ def main():
         "detected_per_photon": detected / args.photons,
         "absorbed_percent": (tot - esc) / tot * 100.0,
     }
+    if not args.no_others:
+        out["other_configs"] = other_configs_leg(sync)
     if not args.no_jacobian:
         log("bench: Jacobian leg")
         out["jacobian"] = jacobian_leg(sync)
