@@ -149,3 +149,24 @@ def test_simulation_without_gpu_fails_loudly():
         U.Simulation(U.config_from_argv(["--bench", "cube60", "-n", "100"]))
     assert U.run_cmd(["--bench", "cube60", "-n", "100"]) == 1
     assert "no HIP device" in U.last_error()
+
+
+def test_reference_structural_cli_checks():
+    """The structural checks of the reference's test/testumcx.sh:13-53 against this build's
+    binary: it exists and is executable, its libraries resolve, the help lists at least five long
+    flags and the cube60 benchmark, and --dumpjson of that benchmark carries its ID."""
+    import re
+    import subprocess
+    umcx = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "umcx_amd", "bin", "umcx")
+    assert os.path.isfile(umcx) and os.access(umcx, os.X_OK)
+    ldd = subprocess.run(["ldd", umcx], capture_output=True, text=True).stdout
+    assert "not found" not in ldd
+    helptext = subprocess.run([umcx], capture_output=True, text=True).stdout
+    assert len(set(re.findall(r"--[a-z]+", helptext))) >= 5
+    assert "cube60" in helptext
+    dump = subprocess.run([umcx, "--bench", "cube60", "--dumpjson"], capture_output=True, text=True).stdout
+    assert re.search(r'"ID":\s*"cube60"', dump)
+    dump = subprocess.run([umcx, "--bench", "cube60", "--json",
+                           '{"Optode":{"Source":{"Type":"isotropic","Pos":[29,29,29]}}}', "--dumpjson"],
+                          capture_output=True, text=True).stdout
+    assert "isotropic" in dump
