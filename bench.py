@@ -324,13 +324,17 @@ SKIN_CONFIG = {
 
 
 def other_configs_leg(sync) -> dict:
-    """Photons/ms of the two other single-GPU configurations: the small homogeneous cube (60^3,
-    reflection off, 1e6 photons, no detectors) and the layered skin model with a vessel
+    """Photons/ms of the other single-GPU configurations: the small homogeneous cube (60^3,
+    reflection off, 1e6 photons, no detectors), the reference's published cube60 benchmark
+    (the same cube with its 4 detectors, 1e8 photons) and the layered skin model with a vessel
     (200^3 voxels of 5 um, Fresnel on, disk source r=60 voxels, 1e8 photons, energy output)."""
     import umcx_amd as U
     out = {}
     cube = U.config_from_argv(["--bench", "cube60", "-d", "0", "-n", "1000000"])
-    for name, cfg, steps in (("cube60_1e6", cube, 5), ("skinvessel_200cube_1e8", SKIN_CONFIG, 1)):
+    # the reference's published benchmark: cube60 (4 detectors on) with 1e8 photons
+    cube_pub = U.config_from_argv(["--bench", "cube60", "-n", "1e8"])
+    for name, cfg, steps in (("cube60_1e6", cube, 5), ("cube60_published_1e8", cube_pub, 1),
+                             ("skinvessel_200cube_1e8", SKIN_CONFIG, 1)):
         with U.Simulation(cfg) as sim:
             n = int(float(cfg["Session"]["Photons"]))
             sim.run()
@@ -343,6 +347,10 @@ def other_configs_leg(sync) -> dict:
             out[name] = {"photons": n, "kernel_ms": round(kms, 3), "photons_per_ms": round(n / kms, 1),
                          "lds_label_bits": sim.info["lds_label_bits"], "tally_box": sim.tally_box(),
                          "absorbed_percent": (tot - esc) / tot * 100.0}
+            if name == "cube60_published_1e8":
+                # webpage/umcx.html:451,455 of the reference: 175,075 photons/ms on an RTX 5090
+                out[name]["published_rtx5090_photons_per_ms"] = 175075
+                out[name]["vs_published_rtx5090"] = round(n / kms / 175075, 3)
         log(f"bench: {name}: {out[name]['photons_per_ms']} photons/ms")
     return out
 
