@@ -158,6 +158,12 @@ void umcx_batch_free(umcx_batch* b);
 int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels);
 /* Expanded config after flags/presets/merges, as --dumpjson prints it (src/umcx.cpp:525-527). */
 int umcx_config_from_argv(char** argv, int argn, char** json_out);
+/* The LDS tally box the sessions choose from a pilot's deposit counts (counts: ngates x nx x ny x
+ * nz, memory order [t][x][y][z]): greedy growth from the densest cell, adding the face slab with
+ * the most counts per added cell while the box holds at most `budget` cells; box = {x0, y0, z0,
+ * gate0, nx, ny, nz, ngates}, all zero for an empty grid. No counterpart in the reference. */
+int umcx_choose_tally_box(const float* counts, int32_t nx, int32_t ny, int32_t nz, int32_t ngates, uint64_t budget,
+                          int32_t box[8]);
 /* glibc srand(seed)+4*rand() words used to seed every photon (src/umcx.cpp:725-727). */
 int umcx_seed_words(int32_t seed, uint32_t out[4]);
 /* Evaluate the libm replicas (fn 0 logf, 1 sinf, 2 cosf, 3 acosf, 4 expf) on the host CPU. */

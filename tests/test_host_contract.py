@@ -151,6 +151,21 @@ def test_simulation_without_gpu_fails_loudly():
     assert "no HIP device" in U.last_error()
 
 
+@pytest.mark.skipif(U.device_count() > 0, reason="checks the no-GPU behaviour")
+def test_batch_without_gpu_fails_loudly():
+    from umcx_amd import batch as B
+    with pytest.raises(U.UmcxError, match="no HIP device"):
+        B.BatchSimulation(B.uneven_jobs(seed=7, njobs=2, max_photons=100))
+
+
+def test_batch_rejects_an_empty_job_list():
+    import ctypes as C
+    from umcx_amd._native import UmcxLaunch, lib
+    h = C.c_void_p()
+    assert lib().umcx_batch_create(None, 0, C.byref(UmcxLaunch()), C.byref(h)) != 0
+    assert "at least one job" in U.last_error()
+
+
 def test_reference_structural_cli_checks():
     """The structural checks of the reference's test/testumcx.sh:13-53 against this build's
     binary: it exists and is executable, its libraries resolve, the help lists at least five long
@@ -170,3 +185,53 @@ def test_reference_structural_cli_checks():
                            '{"Optode":{"Source":{"Type":"isotropic","Pos":[29,29,29]}}}', "--dumpjson"],
                           capture_output=True, text=True).stdout
     assert "isotropic" in dump
+
+
+def _box_sum(c, b):
+    x0, y0, z0, g0, sx, sy, sz, sg = b
+    return c[g0:g0 + sg, x0:x0 + sx, y0:y0 + sy, z0:z0 + sz].sum()
+
+
+def test_tally_box_choice():
+    """The pilot-driven LDS tally box: within budget, contains the densest cell, never worse than
+    the densest cell alone, empty for an empty grid, and on small grids within a few percent of
+    the best box of the same budget found by exhaustive search."""
+    import ctypes as C
+    from umcx_amd._native import check, lib
+
+    def choose(c, budget):
+        nt, nx, ny, nz = c.shape
+        box = (C.c_int32 * 8)()
+        arr = np.ascontiguousarray(c, dtype=np.float32)
+        check(lib().umcx_choose_tally_box(arr.ctypes.data, nx, ny, nz, nt, budget, box))
+        return list(box)
+
+    assert choose(np.zeros((2, 4, 4, 4)), 16) == [0] * 8
+    rng = np.random.default_rng(3)
+    for trial in range(12):
+        nt, nx, ny, nz = 2, 6, 5, 7
+        g = np.indices((nt, nx, ny, nz)).astype(np.float64)
+        centre = rng.uniform([0, 0, 0, 0], [nt, nx, ny, nz])
+        d2 = sum((g[i] - centre[i]) ** 2 for i in range(4))
+        c = (np.exp(-d2 / rng.uniform(1.0, 6.0)) * 1000 + rng.uniform(0, 5, d2.shape)).astype(np.float32)
+        budget = int(rng.integers(8, 60))
+        b = choose(c, budget)
+        x0, y0, z0, g0, sx, sy, sz, sg = b
+        assert 0 < sx * sy * sz * sg <= budget
+        assert 0 <= g0 and g0 + sg <= nt and 0 <= x0 and x0 + sx <= nx and 0 <= y0 and y0 + sy <= ny and 0 <= z0 and z0 + sz <= nz
+        am = np.unravel_index(np.argmax(c), c.shape)
+        assert g0 <= am[0] < g0 + sg and x0 <= am[1] < x0 + sx and y0 <= am[2] < y0 + sy and z0 <= am[3] < z0 + sz
+        got = _box_sum(c, b)
+        best = 0.0
+        for SG in range(1, nt + 1):
+            for SX in range(1, nx + 1):
+                for SY in range(1, ny + 1):
+                    for SZ in range(1, nz + 1):
+                        if SG * SX * SY * SZ > budget:
+                            continue
+                        for G0 in range(nt - SG + 1):
+                            for X0 in range(nx - SX + 1):
+                                for Y0 in range(ny - SY + 1):
+                                    for Z0 in range(nz - SZ + 1):
+                                        best = max(best, _box_sum(c, (X0, Y0, Z0, G0, SX, SY, SZ, SG)))
+        assert got >= 0.9 * best, (trial, got, best)

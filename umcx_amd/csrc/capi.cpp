@@ -12,6 +12,7 @@
 #include <string>
 
 #include "batch.h"
+#include "tally_box.h"
 #include "bjdata.h"
 #include "config.h"
 #include "glibc_libm.h"
@@ -406,6 +407,17 @@ int umcx_config_from_argv(char** argv, int argn, char** json_out) {
         std::string s = in.cfg.dump(2);
         *json_out = (char*)malloc(s.size() + 1);
         memcpy(*json_out, s.c_str(), s.size() + 1);
+        return 0;
+    });
+}
+
+int umcx_choose_tally_box(const float* counts, int32_t nx, int32_t ny, int32_t nz, int32_t ngates, uint64_t budget,
+                          int32_t box[8]) {
+    return guarded([&]() -> int {
+        if (!counts || !box || nx <= 0 || ny <= 0 || nz <= 0 || ngates <= 0)
+            throw std::runtime_error("umcx_choose_tally_box: empty grid or null pointer");
+        const umcx::TallyBox b = umcx::choose_box(counts, nx, ny, nz, ngates, budget);
+        for (int i = 0; i < 8; i++) box[i] = b[i];
         return 0;
     });
 }
