@@ -373,7 +373,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(photon ranges would shard across GPUs with one fluence all-reduce, see DESIGN.md)", file=sys.stderr)
+              "(photon ranges would shard across GPUs with one fluence all-reduce)", file=sys.stderr)
         sys.exit(2)
 
     import umcx_amd as U

@@ -1,7 +1,7 @@
 """GPU parity: the gfx950 kernel (through libumcx.so's C ABI) against the oracle and the
 reference's golden vectors.
 
-Bars (BASELINE north star):
+Bars:
 * per photon: position, direction, scatter count, time of flight, path, medium and weight bitwise
   equal to the oracle;
 * detected-photon rows: bitwise equal to the reference's rows as a multiset;
