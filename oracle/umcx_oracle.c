@@ -1,6 +1,6 @@
 /*
  * umcx_oracle.c -- CPU restatement of umcx's photon loop. TEST INFRASTRUCTURE ONLY:
- * the checker that tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use.
+ * the checker that tests/, the smoke test and bench.py's cpu_baseline leg use.
  * The product never links or calls it.
  *
  * Written from the behaviour of fangq/umcx src/umcx.cpp; each function cites the lines

@@ -1,7 +1,7 @@
 /*
  * umcx_oracle.h -- CPU restatement of umcx's photon loop (TEST INFRASTRUCTURE ONLY).
  *
- * This is the checker, not the product. Only tests/, __graft_entry__.smoke() and the
+ * This is the checker, not the product. Only tests/, the smoke test and the
  * cpu_baseline leg of bench.py may load it. The product path (umcx_amd/ + libumcx.so)
  * never links, loads or calls anything here.
  *
