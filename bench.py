@@ -11,9 +11,10 @@ resident inputs (the label volume and media are uploaded once, outside the timed
 Prints one JSON line. ``roofline`` prices the photon kernel's algorithmic FP32 work (per-photon
 op model of the photon loop, counted from src/umcx.cpp:315-428: 34 per step, 9 per voxel crossing,
 62 per scatter, 1 per random draw, 35 per Fresnel test) against the MI355X FP32 peak, using the
-kernel's HIP-event time on its own stream; ``roofline.valu_issue`` gives the bound the kernel
-actually meets, the VALU instruction issue rate (instructions per photon from rocprofv3 counters)
-with the active-lane fraction. ``cpu_baseline`` times the reference umcx binary built
+kernel's HIP-event time on its own stream. Beside it, ``roofline.atomics`` gives the ceiling
+that binds -- memory-side float atomics per second against the rate umcx_amd/bin/roofline_probe
+measures -- and ``roofline.valu_issue`` the VALU instruction issue rate (instructions per photon
+from rocprofv3 counters) with the active-lane fraction. ``cpu_baseline`` times the reference umcx binary built
 from source (or, when it is absent, the CPU oracle) on this machine's host cores over a bounded
 sample of the same workload.
 """
@@ -416,7 +417,7 @@ def main():
         "config": {"workload": "heterogeneous reflective: 60^3 1 mm voxels, sphere r=15 (n=1.0) in n=1.37 medium, "
                                "Fresnel on, pencil (29.5,29.5,0)+z, 50 gates x 0.1 ns, 4 disk detectors r=1",
                    "photons_per_step": args.photons, "seed": 1648335518},
-        "roofline": {"bound": "valu", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "binding": "atomics", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
                      "traffic": round(HBM_BYTES_PER_PHOTON * args.photons),
                      "traffic_gbps": round(HBM_BYTES_PER_PHOTON * args.photons / (kavg * 1e-3) / 1e9, 1),

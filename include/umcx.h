@@ -58,6 +58,9 @@ typedef struct umcx_info {
     uint32_t seeds[4];
     int32_t block, blocks, tally_mode, label_bytes;
     int32_t lds_label_bits;  /* bits per voxel of the LDS-staged labels (0: labels read from HBM) */
+    int32_t last_block, last_blocks;  /* launch shape of the last run (0 before any): with no explicit
+                                         shape, runs under 12 photons per lane of the full shape use
+                                         half the lanes */
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)

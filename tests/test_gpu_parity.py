@@ -243,6 +243,33 @@ def test_launch_shape_does_not_change_results():
         assert e[1] == outs[0][1][1]
 
 
+@pytest.mark.parametrize("label_cache", [0, -1])
+def test_short_runs_use_half_the_lanes_with_equal_results(label_cache):
+    """A run under 12 photons per lane of the full shape launches half the lanes (half-size
+    workgroups when one workgroup per CU holds the staged labels, else half the workgroups);
+    every photon's final state equals the full shape's bitwise."""
+    g = O.golden("cube60b")
+    n = 3000
+    say(f"short-run shape, label_cache {label_cache}")
+    with U.Simulation(g["cfg"], trace=True, label_cache=label_cache) as sim:
+        block, blocks = sim.info["block"], sim.info["blocks"]
+        sim.run(0, n)
+        short = sim.last_launch()
+        t_short = sim.trace(n)
+    if sim.info["lds_label_bits"]:
+        assert short == (block // 2, blocks)
+    else:
+        assert short == (block, blocks // 2)
+    with U.Simulation(g["cfg"], trace=True, label_cache=label_cache, block=block, blocks=blocks) as sim:
+        sim.run(0, n)
+        assert sim.last_launch() == (block, blocks)
+        t_full = sim.trace(n)
+    exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
+    for key in ("pos", "vec", "len", "w0"):
+        assert np.array_equal(exact(t_short[key]), exact(t_full[key])), key
+    assert np.array_equal(t_short["lastvoxelidx"], t_full["lastvoxelidx"])
+
+
 def test_full_size_heterogeneous_reflective_properties():
     """The 1e8-photon heterogeneous reflective workload at 1e7 photons: energy balance, and the
     detections of a 2e5-photon prefix equal the oracle's bitwise."""
@@ -252,6 +279,7 @@ def test_full_size_heterogeneous_reflective_properties():
     say("full-size c2 1e7")
     with U.Simulation(cfg) as sim:
         sim.run()
+        assert sim.last_launch() == (sim.info["block"], sim.info["blocks"])   # full shape
         esc, tot, dep = sim.energies()
         detected, saved = sim.detected()
         sim.reset()

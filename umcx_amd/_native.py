@@ -24,7 +24,8 @@ class UmcxInfo(C.Structure):
                 ("issavevol", C.c_int32), ("issavedet", C.c_int32), ("isreflect", C.c_int32),
                 ("srctype", C.c_int32), ("tstart", C.c_float), ("tend", C.c_float), ("rtstep", C.c_float),
                 ("unitinmm", C.c_float), ("seeds", C.c_uint32 * 4), ("block", C.c_int32), ("blocks", C.c_int32),
-                ("tally_mode", C.c_int32), ("label_bytes", C.c_int32), ("lds_label_bits", C.c_int32)]
+                ("tally_mode", C.c_int32), ("label_bytes", C.c_int32), ("lds_label_bits", C.c_int32),
+                ("last_block", C.c_int32), ("last_blocks", C.c_int32)]
 
 
 class UmcxBatchTrial(C.Structure):

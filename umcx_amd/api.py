@@ -90,6 +90,12 @@ class Simulation:
         check(lib().umcx_session_reset(self._h))
         self.kernel_ms = 0.0
 
+    def last_launch(self) -> tuple[int, int]:
+        """(lanes per workgroup, workgroups) of the last run's launch."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return info.last_block, info.last_blocks
+
     def run(self, first: int = 0, count: int | None = None) -> float:
         """Run photons [first, first+count); returns the kernel time in ms."""
         n = self.info["nphoton"] if count is None else int(count)

@@ -197,6 +197,7 @@ int umcx_session_info(const umcx_session* s, umcx_info* info) {
         info->block = s->s->block(); info->blocks = s->s->blocks(); info->tally_mode = s->s->tally_mode();
         info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
         info->lds_label_bits = s->s->lds_label_bits();
+        info->last_block = s->s->last_block(); info->last_blocks = s->s->last_blocks();
         return 0;
     });
 }

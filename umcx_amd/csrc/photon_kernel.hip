@@ -37,6 +37,9 @@ namespace umcx {
 constexpr uint64_t kLabelCacheBytes = 96 * 1024;   // largest packed label volume staged in LDS
 constexpr uint64_t kMaxTileCells = 32768;          // largest LDS tally tile
 constexpr int kRegPaths = 4;                       // partial paths kept in registers up to this many media
+// below this many photons per lane of the full launch shape a run ends with its slowest photons,
+// and it runs on half the lanes so that fewer waves share each SIMD
+constexpr uint64_t kShortRunPhotonsPerLane = 12;
 
 namespace dev {
 
@@ -570,6 +573,8 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     if (tally_mode_ == 1 && tile_budget_ < 64) tile_budget_ = 64;
     blocks_ = lc.blocks > 0 ? lc.blocks : prop.multiProcessorCount * per_cu;
     if (p.threadnum > 0 && lc.blocks <= 0) blocks_ = std::max(1, p.threadnum / block_);
+    cu_count_ = std::max(1, prop.multiProcessorCount);
+    auto_shape_ = lc.block <= 0 && lc.blocks <= 0 && p.threadnum <= 0 && p.blocksize <= 0;
     reset();
 }
 
@@ -670,6 +675,19 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     for (int i = 0; i < 8; i++) tile_[i] = box[i];
 }
 
+// Launch shape of one run. The full shape fills every CU; a run of fewer than
+// kShortRunPhotonsPerLane photons per lane is bound by its slowest photons, whose waves issue
+// more often when fewer waves share a SIMD, so it runs on half the lanes: half-size workgroups
+// when one workgroup per CU holds the staged labels (same LDS layout and tally tile), else half
+// the workgroups while at least two per CU remain. Explicit shapes are kept as given.
+void Session::run_shape(uint64_t count, int* block, int* blocks) const {
+    *block = block_;
+    *blocks = blocks_;
+    if (!auto_shape_ || count >= kShortRunPhotonsPerLane * (uint64_t)block_ * (uint64_t)blocks_) return;
+    if (!plabel_words_ && blocks_ >= 2 * cu_count_) *blocks = blocks_ / 2;
+    else if (block_ >= 128) *block = (block_ / 2 / 64) * 64;
+}
+
 RunStats Session::run(uint64_t first, uint64_t count) {
     HIP_OK(hipSetDevice(lc_.device));
     hipStream_t s = (hipStream_t)stream_;
@@ -689,12 +707,17 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     if (!tile_chosen_ && (tally_mode_ == 1 || (tally_mode_ < 0 && count >= 1000000))) choose_tile(first, count);
     dev::KernelArgs ka;
     build_args(first, count, &ka);
+    int rblock = block_, rblocks = blocks_;
+    run_shape(count, &rblock, &rblocks);
+    ka.per_wg = (count - count / 8) / (uint64_t)rblocks;
+    last_block_ = rblock;
+    last_blocks_ = rblocks;
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
     size_t lds = lds_bytes(ka.tile_n);
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds, s);
+    pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0)(ka, rblocks, rblock, lds, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));

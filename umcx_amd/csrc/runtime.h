@@ -74,6 +74,9 @@ public:
     const Problem& problem() const { return prob_; }
     int block() const { return block_; }
     int blocks() const { return blocks_; }
+    // shape of the last run() launch (half the lanes for short runs, see run_shape)
+    int last_block() const { return last_block_; }
+    int last_blocks() const { return last_blocks_; }
     int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
     int lds_label_bits() const { return (int)plabel_bits_; }  // 0 when labels are read from HBM
     void* stream() const { return stream_; }
@@ -84,6 +87,8 @@ private:
     Problem prob_;
     LaunchConfig lc_;
     int block_ = 256, blocks_ = 0, tally_mode_ = 0;
+    int cu_count_ = 1, last_block_ = 0, last_blocks_ = 0;
+    bool auto_shape_ = false;            // launch shape chosen per run (no explicit block, blocks, threads)
     int label_bytes_ = 1;
     uint64_t fluence_len_ = 0;
     void* stream_ = nullptr;
@@ -116,6 +121,7 @@ private:
     void* build_args(uint64_t first, uint64_t count, void* out) const;
     size_t lds_bytes(uint32_t tile_n, bool paths_needed = true) const;   // dynamic LDS of a launch
     void choose_tile(uint64_t first, uint64_t count);
+    void run_shape(uint64_t count, int* block, int* blocks) const;
 
 public:
     // the chosen LDS tally tile, {x0, y0, z0, g0, sx, sy, sz, sg} (all zero when none)
