@@ -64,7 +64,8 @@ typedef struct umcx_info {
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)
- * or from command-line arguments. Uploads the domain to the GPU. */
+ * or from command-line arguments (argn < 0: argv[0] is a BJData config buffer of -argn bytes,
+ * :475-478). Uploads the domain to the GPU. */
 int umcx_session_from_json(const char* json, const umcx_launch* launch, umcx_session** out);
 int umcx_session_from_argv(char** argv, int argn, const umcx_launch* launch, umcx_session** out);
 int umcx_session_info(const umcx_session* s, umcx_info* info);
@@ -159,7 +160,8 @@ void umcx_batch_free(umcx_batch* b);
 /* Label volume with detector bits, as MCX_userio::initdomain + maskdetectors build it
  * (src/umcx.cpp:537-632). *labels is allocated by the library; release with umcx_free. */
 int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels);
-/* Expanded config after flags/presets/merges, as --dumpjson prints it (src/umcx.cpp:525-527). */
+/* Expanded config after flags/presets/merges, as --dumpjson prints it (src/umcx.cpp:525-527);
+ * with argn < 0, the JSON form of the BJData config buffer argv[0] (-argn bytes). */
 int umcx_config_from_argv(char** argv, int argn, char** json_out);
 /* The LDS tally box the sessions choose from a pilot's deposit counts (counts: ngates x nx x ny x
  * nz, memory order [t][x][y][z]): greedy growth from the densest cell, adding the face slab with
@@ -177,8 +179,10 @@ void umcx_free(void* p);
 
 #ifdef __cplusplus
 }
-/* C++ linkage, as in the reference (src/umcx.cpp:783). argn < 0 (a BJData buffer from MATLAB)
- * is rejected with an error. */
+/* C++ linkage, as in the reference (src/umcx.cpp:783). argn < 0: argv[0] is a BJData buffer of
+ * -argn bytes holding the whole config, as the reference's MATLAB binding passes it
+ * (src/umcx.cpp:475-478, :860-865); nlhs/plhs (MATLAB outputs) are ignored and the output
+ * files are written as for the other inputs. */
 int MCX_run_simulation(char* argv[], int argn = 1, int nlhs = 0, void* plhs[] = nullptr);
 #endif
 

@@ -4,7 +4,7 @@ The product is libumcx.so (C ABI in include/umcx.h): hand-written gfx950 HIP ker
 umcx's JSON/JData contract and its MCX_run_cmd / MCX_run_json exports. This package is its
 Python front end.
 """
-from .api import (Result, Simulation, UmcxError, config_from_argv, device_count, domain, last_error,  # noqa: F401
-                  libm_eval, run_cmd, run_json, seed_words, simulate)
+from .api import (Result, Simulation, UmcxError, config_from_argv, config_from_bjdata, device_count,  # noqa: F401
+                  domain, last_error, libm_eval, run_bjdata, run_cmd, run_json, seed_words, simulate)
 
 __version__ = "0.1.0"

@@ -37,6 +37,8 @@ class UmcxBatchTrial(C.Structure):
 SIGNATURES = {
     "MCX_run_cmd": (C.c_int, [C.POINTER(C.c_char_p), C.c_int]),
     "MCX_run_json": (C.c_int, [C.c_char_p]),
+    # C++ linkage entry of src/umcx.cpp:783, MCX_run_simulation(char**, int, int, void**)
+    "_Z18MCX_run_simulationPPciiPPv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_void_p]),
     "umcx_last_error": (C.c_char_p, []),
     "umcx_version": (C.c_char_p, []),
     "umcx_device_count": (C.c_int, []),
@@ -121,6 +123,16 @@ def lib() -> C.CDLL:
 def check(rc: int) -> None:
     if rc != 0:
         raise UmcxError(lib().umcx_last_error().decode("utf-8", "replace"))
+
+
+def buffer_argv(buf: bytes):
+    """argv for argn < 0: argv[0] points at a BJData buffer (the array keeps `buf` alive)."""
+    if not isinstance(buf, (bytes, bytearray)) or len(buf) == 0:
+        raise UmcxError("a BJData input needs a non-empty bytes buffer")
+    arr = (C.c_char_p * 2)()
+    arr[0] = bytes(buf)
+    arr[1] = None
+    return arr
 
 
 def argv_array(args: list[str]):

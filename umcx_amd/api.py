@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._native import UmcxError, UmcxInfo, UmcxLaunch, argv_array, check, lib
+from ._native import UmcxError, UmcxInfo, UmcxLaunch, argv_array, buffer_argv, check, lib
 
 TRACE_DTYPE = np.dtype([("pos", "<f4", 4), ("vec", "<f4", 4), ("len", "<f4", 4), ("mediaid", "<i4"),
                         ("lastvoxelidx", "<i4"), ("w0", "<f4"), ("pad", "<u4")])
@@ -50,11 +50,14 @@ class Simulation:
     """One prepared problem on the GPU."""
 
     def __init__(self, cfg=None, argv: list[str] | None = None, device: int = 0, block: int = 0,
-                 blocks: int = 0, tally_mode: int = -1, trace: bool = False, vote: int = -1, label_cache: int = 0):
+                 blocks: int = 0, tally_mode: int = -1, trace: bool = False, vote: int = -1, label_cache: int = 0,
+                 bjdata: bytes | None = None):
         L = lib()
         launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote, label_cache)
         h = C.c_void_p()
-        if argv is not None:
+        if bjdata is not None:      # a BJData config buffer, as MATLAB passes it (src/umcx.cpp:475-478)
+            check(L.umcx_session_from_argv(buffer_argv(bjdata), -len(bjdata), C.byref(launch), C.byref(h)))
+        elif argv is not None:
             args = ["umcx"] + list(argv)
             check(L.umcx_session_from_argv(argv_array(args), len(args), C.byref(launch), C.byref(h)))
         else:
@@ -185,6 +188,23 @@ def run_cmd(args: list[str]) -> int:
 def run_json(cfg) -> int:
     """The reference's MCX_run_json: one JSON document as the whole configuration."""
     return lib().MCX_run_json(_cfg_text(cfg).encode())
+
+
+def run_bjdata(buf: bytes) -> int:
+    """The reference's MCX_run_simulation with argn < 0: argv[0] is a BJData config buffer (the
+    MATLAB input, src/umcx.cpp:475-478); runs it and writes the output files."""
+    argv = buffer_argv(buf)
+    return lib()._Z18MCX_run_simulationPPciiPPv(argv, -len(buf), 0, None)
+
+
+def config_from_bjdata(buf: bytes) -> dict:
+    """The configuration a BJData input buffer holds, as umcx reads it (no GPU needed)."""
+    L = lib()
+    out = C.c_void_p()
+    check(L.umcx_config_from_argv(buffer_argv(buf), -len(buf), C.byref(out)))
+    text = C.cast(out, C.c_char_p).value.decode("utf-8", "surrogateescape")   # strings are raw bytes
+    L.umcx_free(out)
+    return json.loads(text)
 
 
 def last_error() -> str:

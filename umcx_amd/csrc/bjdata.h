@@ -32,4 +32,14 @@ void write_detp(const std::string& path, const DetpInfo& info, const float* rows
 // exposed for tests: serialise one unsigned/signed integer with its smallest marker
 std::string bj_int(int64_t v);
 
+// Reader for the BJData input buffer that MCX_run_simulation takes when argn < 0
+// (src/umcx.cpp:475-478: json::from_bjdata of a MATLAB-encoded config). Decodes BJData draft 2
+// as the reference's JSON library does: little-endian scalars (i U I u l m L M h d D C S H, Z T F,
+// N no-op), containers with optional '$' type and '#' count, and N-D arrays ('#' followed by a
+// dimension vector), which become {"_ArraySize_", "_ArrayType_", "_ArrayData_"} objects (a 1-D or
+// 1 x n dimension vector gives a plain array, a zero dimension an empty one). Structure-of-arrays
+// containers ('$' '{') are rejected. Throws std::runtime_error on malformed or truncated input.
+class Json;
+Json bjdata_decode(const uint8_t* buf, size_t n);
+
 }  // namespace umcx

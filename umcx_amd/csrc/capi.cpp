@@ -76,7 +76,6 @@ int MCX_run_simulation(char* argv[], int argn, int nlhs, void* plhs[]) {
     (void)nlhs;
     (void)plhs;
     return guarded([&]() -> int {
-        if (argn < 0) throw std::runtime_error("BJData (MATLAB) input buffers are not supported by this build");
         umcx::ParsedInput in = umcx::parse_argv(argn, argv);
         if (in.action == umcx::Action::Help) {
             std::cout << umcx::help_text() << std::flush;

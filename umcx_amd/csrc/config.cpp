@@ -1,6 +1,8 @@
 // config.cpp -- umcx's input contract (see config.h).
 #include "config.h"
 
+#include "bjdata.h"
+
 #include <zlib.h>
 
 #include <cmath>
@@ -283,6 +285,11 @@ Json benchmark_config(const std::string& name) {
 
 ParsedInput parse_argv(int argc, char** argv) {
     ParsedInput in;
+    if (argc < 0) {   // argv[0] is a BJData buffer of -argc bytes, as MATLAB passes it (:475-478)
+        if (!argv || !argv[0]) fail("BJData input: null buffer");
+        in.cfg = bjdata_decode((const uint8_t*)argv[0], (size_t)(-(int64_t)argc));
+        return in;
+    }
     if (argc <= 1) { in.action = Action::Help; return in; }
     std::vector<std::string> params(argv + 1, argv + argc);
     bool want_help = false;
@@ -390,8 +397,20 @@ void build_domain(const Json& cfg, uint32_t& nx, uint32_t& ny, uint32_t& nz, std
                 if (data->size() < vol.size()) fail("_ArrayData_ holds fewer elements than _ArraySize_ describes");
                 for (size_t i = 0; i < vol.size(); i++) vol[i] = (int32_t)data->val(i).as_int();
             } else if (const Json* z = s.find("_ArrayZipData_")) {
-                if (!z->is_string()) fail("_ArrayZipData_ must be a base64 string in JSON input");
-                std::vector<uint8_t> comp = base64_decode(z->as_string());
+                // a base64 string (JSON text) or an array of byte values (BJData input, :584)
+                std::vector<uint8_t> comp;
+                if (z->is_string()) {
+                    comp = base64_decode(z->as_string());
+                } else if (z->is_array()) {
+                    comp.resize(z->size());
+                    for (size_t i = 0; i < z->size(); i++) {
+                        const uint64_t b = z->at(i).as_uint();
+                        if (b > 255) fail("_ArrayZipData_ byte values must be within 0..255");
+                        comp[i] = (uint8_t)b;
+                    }
+                } else {
+                    fail("_ArrayZipData_ must be a base64 string or a byte array");
+                }
                 std::vector<uint8_t> raw = inflate_bytes(comp, s.value_str("_ArrayZipType_", "zlib") == "gzip");
                 if (dtype == "uint8") copy_cast<uint8_t>(raw, vol);
                 else if (dtype == "int8") copy_cast<int8_t>(raw, vol);
