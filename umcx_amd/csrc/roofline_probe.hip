@@ -4,7 +4,8 @@
 //            32 waves per SIMD over the run), in wave instructions per second;
 //   atomics: device-scope float atomic adds (no return) to hashed addresses in a 40 MiB buffer
 //            (the size of a time-gated fluence grid), in atomics per second, and the same into
-//            one hot 256-float region (same-address serialisation at the memory side).
+//            one hot 256-float region (same-address serialisation at the memory side), and the
+//            hashed rate over footprints from 64 KiB to 1 GiB.
 //
 // Prints one JSON line. Every index is taken modulo the buffer size; the launches are bounded.
 #include <hip/hip_runtime.h>
@@ -72,15 +73,23 @@ int main() {
 
     // atomics: 4096 workgroups x 256 lanes x 256 adds
     const uint32_t n = 10u << 20, ablocks = 4096, athreads = 256, aiters = 256;
+    const uint32_t nmax = 256u << 20;     // the footprint sweep goes up to 1 GiB of floats
     float* buf;
-    CK(hipMalloc(&buf, (size_t)n * 4));
-    CK(hipMemset(buf, 0, (size_t)n * 4));
+    CK(hipMalloc(&buf, (size_t)nmax * 4));
+    CK(hipMemset(buf, 0, (size_t)nmax * 4));
     auto spread = [&] { hipLaunchKernelGGL(spread_atomics, dim3(ablocks), dim3(athreads), 0, 0, buf, n, aiters); };
     auto hot = [&] { hipLaunchKernelGGL(spread_atomics, dim3(ablocks), dim3(athreads), 0, 0, buf, 256u, aiters); };
     time_ms(spread);
     const double sms = time_ms(spread);
     const double hms = time_ms(hot);
     const double natom = (double)ablocks * athreads * aiters;
+    const uint32_t fps[] = {16u << 10, 256u << 10, 1u << 20, 4u << 20, 40u << 20, 100u << 20, 256u << 20};
+    double fp_rate[sizeof fps / sizeof fps[0]];
+    for (size_t i = 0; i < sizeof fps / sizeof fps[0]; i++) {
+        auto run = [&] { hipLaunchKernelGGL(spread_atomics, dim3(ablocks), dim3(athreads), 0, 0, buf, fps[i], aiters); };
+        time_ms(run);
+        fp_rate[i] = natom / (time_ms(run) * 1e-3);
+    }
 
     hipDeviceProp_t prop;
     CK(hipGetDeviceProperties(&prop, 0));
@@ -88,6 +97,10 @@ int main() {
            "\"spread_atomics_per_s\": %.4e, \"hot_atomics_per_s\": %.4e}\n",
            prop.gcnArchName, prop.multiProcessorCount, winstr / (fms * 1e-3), fms, natom / (sms * 1e-3),
            natom / (hms * 1e-3));
+    printf("{\"spread_atomics_per_s_by_footprint_mib\": {");
+    for (size_t i = 0; i < sizeof fps / sizeof fps[0]; i++)
+        printf("%s\"%g\": %.4e", i ? ", " : "", fps[i] * 4.0 / (1 << 20), fp_rate[i]);
+    printf("}}\n");
     CK(hipFree(fout));
     CK(hipFree(buf));
     return 0;
