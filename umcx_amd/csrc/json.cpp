@@ -222,6 +222,59 @@ void format_double(std::string& out, double d) {
 
 }  // namespace
 
+struct Json::Heap {
+    std::string s;
+    std::vector<std::string> keys;   // object keys (Object) -- parallel to vals
+    std::vector<Json> vals;          // array elements (Array) or object values (Object)
+};
+
+const Json::Heap& Json::heap() const {
+    static const Heap empty;
+    return h_ ? *h_ : empty;
+}
+Json::Heap& Json::heap_mut() {
+    if (!h_) h_ = new Heap;
+    return *h_;
+}
+Json::Json(const char* s) : type_(Type::String) { heap_mut().s = s; }
+Json::Json(std::string s) : type_(Type::String) { heap_mut().s = std::move(s); }
+// the scalar union is copied as raw bytes, whichever member is active
+Json::Json(const Json& o) : type_(o.type_), h_(o.h_ ? new Heap(*o.h_) : nullptr) { memcpy(&i_, &o.i_, sizeof i_); }
+Json::Json(Json&& o) noexcept : type_(o.type_), h_(o.h_) {
+    memcpy(&i_, &o.i_, sizeof i_);
+    o.h_ = nullptr;
+    o.type_ = Type::Null;
+}
+Json& Json::operator=(const Json& o) {
+    if (this != &o) {
+        Heap* h = o.h_ ? new Heap(*o.h_) : nullptr;   // copy first: o may live inside this
+        delete h_;
+        h_ = h;
+        type_ = o.type_;
+        memcpy(&i_, &o.i_, sizeof i_);
+    }
+    return *this;
+}
+Json& Json::operator=(Json&& o) noexcept {
+    if (this != &o) {
+        Heap* h = o.h_;
+        const Type t = o.type_;
+        int64_t bits;
+        memcpy(&bits, &o.i_, sizeof bits);
+        o.h_ = nullptr;
+        o.type_ = Type::Null;
+        delete h_;                                     // o may live inside this: taken out above
+        h_ = h;
+        type_ = t;
+        memcpy(&i_, &bits, sizeof bits);
+    }
+    return *this;
+}
+const std::string& Json::key(size_t i) const { return heap().keys.at(i); }
+const Json& Json::val(size_t i) const { return heap().vals.at(i); }
+Json& Json::val(size_t i) { return heap_mut().vals.at(i); }
+Json::~Json() { delete h_; }
+
 Json Json::parse(const std::string& text) {
     Parser ps{text.data(), text.data() + text.size()};
     Json v = ps.value(0);
@@ -259,21 +312,22 @@ uint64_t Json::as_uint() const {
 }
 const std::string& Json::as_string() const {
     if (type_ != Type::String) throw JsonError("[json.exception.type_error.302] type must be string");
-    return s_;
+    return heap().s;
 }
 
 size_t Json::size() const {
-    if (type_ == Type::Array || type_ == Type::Object) return vals_.size();
+    if (type_ == Type::Array || type_ == Type::Object) return heap().vals.size();
     return type_ == Type::Null ? 0 : 1;
 }
 const Json* Json::find(const std::string& k) const {
     if (type_ != Type::Object) return nullptr;
-    for (size_t i = 0; i < keys_.size(); i++) if (keys_[i] == k) return &vals_[i];
+    const Heap& h = heap();
+    for (size_t i = 0; i < h.keys.size(); i++) if (h.keys[i] == k) return &h.vals[i];
     return nullptr;
 }
 Json* Json::find(const std::string& k) {
-    if (type_ != Type::Object) return nullptr;
-    for (size_t i = 0; i < keys_.size(); i++) if (keys_[i] == k) return &vals_[i];
+    if (type_ != Type::Object || !h_) return nullptr;
+    for (size_t i = 0; i < h_->keys.size(); i++) if (h_->keys[i] == k) return &h_->vals[i];
     return nullptr;
 }
 bool Json::contains(const std::string& k) const { return find(k) != nullptr; }
@@ -284,27 +338,30 @@ const Json& Json::at(const std::string& k) const {
 }
 const Json& Json::at(size_t i) const {
     if (type_ != Type::Array && type_ != Type::Object) throw JsonError("[json.exception.type_error.304] cannot use at() with a scalar");
-    if (i >= vals_.size()) throw JsonError("[json.exception.out_of_range.401] array index " + std::to_string(i) + " is out of range");
-    return vals_[i];
+    const Heap& h = heap();
+    if (i >= h.vals.size()) throw JsonError("[json.exception.out_of_range.401] array index " + std::to_string(i) + " is out of range");
+    return h.vals[i];
 }
 Json& Json::operator[](const std::string& k) {
     if (type_ == Type::Null) type_ = Type::Object;
     if (type_ != Type::Object) throw JsonError("[json.exception.type_error.305] cannot use operator[] with a string argument");
-    for (size_t i = 0; i < keys_.size(); i++) if (keys_[i] == k) return vals_[i];
-    keys_.push_back(k);
-    vals_.emplace_back();
-    return vals_.back();
+    Heap& h = heap_mut();
+    for (size_t i = 0; i < h.keys.size(); i++) if (h.keys[i] == k) return h.vals[i];
+    h.keys.push_back(k);
+    h.vals.emplace_back();
+    return h.vals.back();
 }
 Json& Json::operator[](size_t i) {
     if (type_ == Type::Null) type_ = Type::Array;
     if (type_ != Type::Array) throw JsonError("[json.exception.type_error.305] cannot use operator[] with a numeric argument");
-    if (i >= vals_.size()) vals_.resize(i + 1);
-    return vals_[i];
+    Heap& h = heap_mut();
+    if (i >= h.vals.size()) h.vals.resize(i + 1);
+    return h.vals[i];
 }
 void Json::push_back(Json v) {
     if (type_ == Type::Null) type_ = Type::Array;
     if (type_ != Type::Array) throw JsonError("[json.exception.type_error.308] cannot use push_back() with a non-array");
-    vals_.push_back(std::move(v));
+    heap_mut().vals.push_back(std::move(v));
 }
 
 double Json::value(const std::string& k, double dflt) const {
@@ -324,12 +381,13 @@ void Json::update(const Json& other) {
     if (!other.is_object()) throw JsonError("[json.exception.type_error.312] cannot use update() with a non-object");
     if (type_ == Type::Null) type_ = Type::Object;
     if (type_ != Type::Object) throw JsonError("[json.exception.type_error.312] cannot use update() with a non-object");
-    for (size_t i = 0; i < other.keys_.size(); i++) {
-        Json* mine = find(other.keys_[i]);
-        if (mine && mine->is_object() && other.vals_[i].is_object()) {
-            mine->update(other.vals_[i]);
+    const Heap& oh = other.heap();
+    for (size_t i = 0; i < oh.keys.size(); i++) {
+        Json* mine = find(oh.keys[i]);
+        if (mine && mine->is_object() && oh.vals[i].is_object()) {
+            mine->update(oh.vals[i]);
         } else {
-            (*this)[other.keys_[i]] = other.vals_[i];
+            (*this)[oh.keys[i]] = oh.vals[i];
         }
     }
 }
@@ -363,21 +421,22 @@ void Json::dump_to(std::string& out, int indent, int level) const {
         case Type::Int: snprintf(buf, sizeof buf, "%lld", (long long)i_); out += buf; return;
         case Type::UInt: snprintf(buf, sizeof buf, "%llu", (unsigned long long)u_); out += buf; return;
         case Type::Float: format_double(out, d_); return;
-        case Type::String: escape(out, s_); return;
+        case Type::String: escape(out, heap().s); return;
         case Type::Array:
         case Type::Object: {
             bool isobj = type_ == Type::Object;
+            const Heap& h = heap();
             out += isobj ? '{' : '[';
-            if (vals_.empty()) { out += isobj ? '}' : ']'; return; }
+            if (h.vals.empty()) { out += isobj ? '}' : ']'; return; }
             std::string pad = indent >= 0 ? std::string((size_t)indent * (level + 1), ' ') : std::string();
-            for (size_t i = 0; i < vals_.size(); i++) {
+            for (size_t i = 0; i < h.vals.size(); i++) {
                 if (i) out += ',';
                 if (indent >= 0) { out += '\n'; out += pad; }
                 if (isobj) {
-                    escape(out, keys_[i]);
+                    escape(out, h.keys[i]);
                     out += indent >= 0 ? ": " : ":";
                 }
-                vals_[i].dump_to(out, indent, level + 1);
+                h.vals[i].dump_to(out, indent, level + 1);
             }
             if (indent >= 0) { out += '\n'; out += std::string((size_t)indent * level, ' '); }
             out += isobj ? '}' : ']';

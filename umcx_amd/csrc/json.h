@@ -32,8 +32,13 @@ public:
     Json(unsigned long v) : type_(Type::UInt), u_(v) {}
     Json(unsigned long long v) : type_(Type::UInt), u_(v) {}
     Json(double v) : type_(Type::Float), d_(v) {}
-    Json(const char* s) : type_(Type::String), s_(s) {}
-    Json(std::string s) : type_(Type::String), s_(std::move(s)) {}
+    Json(const char* s);
+    Json(std::string s);
+    Json(const Json& o);
+    Json(Json&& o) noexcept;
+    Json& operator=(const Json& o);
+    Json& operator=(Json&& o) noexcept;
+    ~Json();
 
     static Json array() { Json j; j.type_ = Type::Array; return j; }
     static Json object() { Json j; j.type_ = Type::Object; return j; }
@@ -65,9 +70,9 @@ public:
     void push_back(Json v);
 
     // object iteration in insertion order
-    const std::string& key(size_t i) const { return keys_.at(i); }
-    const Json& val(size_t i) const { return vals_.at(i); }
-    Json& val(size_t i) { return vals_.at(i); }
+    const std::string& key(size_t i) const;
+    const Json& val(size_t i) const;
+    Json& val(size_t i);
 
     // value(key, default): missing or null keys give the default
     double value(const std::string& key, double dflt) const;
@@ -82,14 +87,20 @@ public:
 private:
     void dump_to(std::string& out, int indent, int level) const;
 
+    // 24 bytes per value, so that large arrays (label volumes as _ArrayData_) stay compact: the
+    // scalar in a union, strings and containers behind one pointer
+    struct Heap;
+    const Heap& heap() const;   // an empty heap when none is allocated
+    Heap& heap_mut();           // allocates on first use
+
     Type type_ = Type::Null;
-    bool b_ = false;
-    int64_t i_ = 0;
-    uint64_t u_ = 0;
-    double d_ = 0.0;
-    std::string s_;
-    std::vector<std::string> keys_;   // object keys (Object) -- parallel to vals_
-    std::vector<Json> vals_;          // array elements (Array) or object values (Object)
+    union {
+        int64_t i_ = 0;
+        uint64_t u_;
+        double d_;
+        bool b_;
+    };
+    Heap* h_ = nullptr;         // String: s; Array: vals; Object: keys and vals (parallel)
 };
 
 }  // namespace umcx
