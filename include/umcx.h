@@ -46,6 +46,8 @@ typedef struct umcx_launch {
     int32_t vote;        /* scatter-round threshold in lanes; 0 scatters as soon as a path ends, -1 tuned */
     int32_t label_cache; /* 0: media ids packed into LDS when the volume fits (<= 96 KiB at 2 or 4 bits
                             per voxel), -1: labels always read from HBM */
+    int32_t deposit_cache; /* 0: per-lane cache of the last 2 global fluence deposits when the LDS tile
+                              takes under half of them (pilot count), 1: always, -1: never */
 } umcx_launch;
 
 /* Constants of a prepared problem: MCX_param (src/umcx.cpp:95-99) plus launch facts. */
@@ -61,6 +63,7 @@ typedef struct umcx_info {
     int32_t last_block, last_blocks;  /* launch shape of the last run (0 before any): with no explicit
                                          shape, runs under 12 photons per lane of the full shape use
                                          half the lanes */
+    int32_t deposit_cache;            /* 1 when runs use the per-lane deposit cache */
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)

@@ -270,6 +270,25 @@ def test_short_runs_use_half_the_lanes_with_equal_results(label_cache):
     assert np.array_equal(t_short["lastvoxelidx"], t_full["lastvoxelidx"])
 
 
+def test_deposit_cache_changes_only_summation_order():
+    """The per-lane cache of global deposits (on, off): same detections and energies, fluence
+    equal up to float summation order (float32 sums of ~1e5 deposits in the hottest cells differ
+    by ~1e-5 relative between orders, so the bar is the fluence bar, 1e-4 relative L2)."""
+    g = O.golden("c2")
+    n = 200000
+    say("deposit cache on/off c2")
+    out = {}
+    for mode in (1, -1):
+        with U.Simulation(g["cfg"], deposit_cache=mode, tally_mode=1) as sim:
+            assert sim.deposit_cache() == (mode == 1)
+            sim.run(0, n)
+            out[mode] = (sorted_rows(sim.detp()[0]), sim.energies(), sim.fluence().astype(np.float64))
+    assert np.array_equal(out[1][0].view(np.uint32), out[-1][0].view(np.uint32))
+    assert out[1][1][1] == out[-1][1][1]
+    a, b = out[1][2], out[-1][2]
+    assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b)
+
+
 def test_full_size_heterogeneous_reflective_properties():
     """The 1e8-photon heterogeneous reflective workload at 1e7 photons: energy balance, and the
     detections of a 2e5-photon prefix equal the oracle's bitwise."""
@@ -280,6 +299,7 @@ def test_full_size_heterogeneous_reflective_properties():
     with U.Simulation(cfg) as sim:
         sim.run()
         assert sim.last_launch() == (sim.info["block"], sim.info["blocks"])   # full shape
+        assert sim.deposit_cache()            # the LDS tile takes under half of the deposits here
         esc, tot, dep = sim.energies()
         detected, saved = sim.detected()
         sim.reset()

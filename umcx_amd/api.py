@@ -51,9 +51,9 @@ class Simulation:
 
     def __init__(self, cfg=None, argv: list[str] | None = None, device: int = 0, block: int = 0,
                  blocks: int = 0, tally_mode: int = -1, trace: bool = False, vote: int = -1, label_cache: int = 0,
-                 bjdata: bytes | None = None):
+                 bjdata: bytes | None = None, deposit_cache: int = 0):
         L = lib()
-        launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote, label_cache)
+        launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote, label_cache, deposit_cache)
         h = C.c_void_p()
         if bjdata is not None:      # a BJData config buffer, as MATLAB passes it (src/umcx.cpp:475-478)
             check(L.umcx_session_from_argv(buffer_argv(bjdata), -len(bjdata), C.byref(launch), C.byref(h)))
@@ -98,6 +98,12 @@ class Simulation:
         info = UmcxInfo()
         check(lib().umcx_session_info(self._h, C.byref(info)))
         return info.last_block, info.last_blocks
+
+    def deposit_cache(self) -> bool:
+        """Whether runs use the per-lane cache of global fluence deposits."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return bool(info.deposit_cache)
 
     def run(self, first: int = 0, count: int | None = None) -> float:
         """Run photons [first, first+count); returns the kernel time in ms."""

@@ -54,6 +54,7 @@ umcx::LaunchConfig to_launch(const umcx_launch* l) {
         lc.trace = l->trace != 0;
         lc.vote = l->vote;
         lc.label_cache = l->label_cache;
+        lc.deposit_cache = l->deposit_cache;
     }
     return lc;
 }
@@ -197,6 +198,7 @@ int umcx_session_info(const umcx_session* s, umcx_info* info) {
         info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
         info->lds_label_bits = s->s->lds_label_bits();
         info->last_block = s->s->last_block(); info->last_blocks = s->s->last_blocks();
+        info->deposit_cache = s->s->deposit_cache() ? 1 : 0;
         return 0;
     });
 }

@@ -332,6 +332,47 @@ __device__ __forceinline__ bool boundary(Photon& p, float n1, float n2, Rng& r, 
     return false;
 }
 
+// Per-lane write-back cache of the last K global deposits, most recent first. A photon that
+// steps back into a voxel it left a moment ago (same time gate) adds to the cached partial sum
+// instead of issuing another memory-side atomic; an entry reaches the grid when it is evicted or
+// flushed. Only the float summation order of a cell changes.
+constexpr uint64_t kNoKey = ~0ull;
+__device__ __forceinline__ uint64_t in_reg64(uint64_t x) {
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
+template <int K>
+__device__ __forceinline__ void cached_deposit(float* grid, uint64_t (&key)[K], float (&sum)[K], uint64_t idx,
+                                               float val) {
+    int pos = K;
+#pragma unroll
+    for (int j = K - 1; j >= 0; j--) pos = key[j] == idx ? j : pos;
+    float cur = 0.f;
+#pragma unroll
+    for (int j = 0; j < K; j++) cur = pos == j ? in_reg(sum[j]) : cur;
+    if (pos == K) {                         // miss: the least recent entry goes to the grid
+        if (key[K - 1] != kNoKey) unsafeAtomicAdd(grid + key[K - 1], sum[K - 1]);
+        pos = K - 1;
+    }
+#pragma unroll
+    for (int j = K - 1; j >= 1; j--) {      // entries above pos move down one place
+        const bool sh = j <= pos;
+        key[j] = sh ? in_reg64(key[j - 1]) : in_reg64(key[j]);
+        sum[j] = sh ? in_reg(sum[j - 1]) : in_reg(sum[j]);
+    }
+    key[0] = idx;
+    sum[0] = cur + val;
+}
+template <int K>
+__device__ __forceinline__ void flush_deposits(float* grid, uint64_t (&key)[K], float (&sum)[K]) {
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+        if (key[j] != kNoKey) unsafeAtomicAdd(grid + key[j], sum[j]);
+        key[j] = kNoKey;
+        sum[j] = 0.f;
+    }
+}
+
 __device__ __forceinline__ unsigned long long lanemask_lt() {
     return (1ull << __lane_id()) - 1ull;
 }

@@ -40,6 +40,8 @@ constexpr int kRegPaths = 4;                       // partial paths kept in regi
 // below this many photons per lane of the full launch shape a run ends with its slowest photons,
 // and it runs on half the lanes so that fewer waves share each SIMD
 constexpr uint64_t kShortRunPhotonsPerLane = 12;
+constexpr int kDepCache = 2;                       // global deposits cached per lane
+constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 
 namespace dev {
 
@@ -67,6 +69,7 @@ struct KernelArgs {
     int32_t tx0, ty0, tz0, tg0;
     uint32_t tsx, tsy, tsz, tsg, tcells, tile_n;
     int32_t count_mode;      // 1: tally the number of deposits instead of their values
+    int32_t dep_cache;       // 1: global deposits go through the per-lane write-back cache
     int32_t vote;            // 0: scatter right after the step that ends a free path; k > 0: a wave
                              // scatters only once k of its lanes (or all of them) wait to scatter
     // per-voxel {mua, mus} in per-voxel-length units overriding the labels' (nullptr: labels only)
@@ -170,6 +173,13 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     float pp0 = 0.f, pp1 = 0.f, pp2 = 0.f, pp3 = 0.f;
     uint64_t rbase = 0, rk = 0;
     const bool savevol = !REPLAY && a.issavevol != 0;
+    uint64_t dkey[kDepCache];
+    float dsum[kDepCache];
+#pragma unroll
+    for (int j = 0; j < kDepCache; j++) {
+        dkey[j] = kNoKey;
+        dsum[j] = 0.f;
+    }
     const bool energy_out = a.outputtype == kEnergy;
 
     while (true) {
@@ -289,6 +299,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                         if (p.tvox >= 0 && g < a.tsg)
                             __hip_atomic_fetch_add(stile + g * a.tcells + (unsigned)p.tvox, val, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+                        else if (a.dep_cache)
+                            cached_deposit(a.fluence, dkey, dsum, (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
                         else
                             unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
                     }
@@ -369,6 +381,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         }
         alive = false;
     }
+
+    if (savevol) flush_deposits(a.fluence, dkey, dsum);
 
     // ---- flush the LDS tally tile ----
     if (a.tile_n) {
@@ -623,6 +637,7 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     ka.cursor = d_cursor_;
     ka.trace = lc_.trace ? d_trace_ : nullptr;
     ka.vote = lc_.vote < 0 ? 0 : std::min(lc_.vote, 64);
+    ka.dep_cache = deposit_cache() ? 1 : 0;
     ka.vprops = (const float2*)d_vprops_;
     ka.unit = p.unitinmm;
     ka.plabels = d_plabels_;
@@ -673,6 +688,9 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
     const TallyBox box = choose_box(cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate, tile_budget_);
     for (int i = 0; i < 8; i++) tile_[i] = box[i];
+    // the per-lane deposit cache pays where most deposits go to HBM; when the tile takes most of
+    // them its extra work per deposit costs more than the atomics it saves
+    dep_cache_ = box_share(CountGrid{cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate}, box) < kCacheBelowShare;
 }
 
 // Launch shape of one run. The full shape fills every CU; a run of fewer than
@@ -753,6 +771,7 @@ void Session::set_voxel_props(const float* mua, const float* mus, bool on_device
     if (staging) HIP_OK(hipFree(staging));
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
+    dep_cache_ = true;
 }
 
 void Session::set_sources(const float* table, int nsrc, uint64_t per_source) {
@@ -768,6 +787,7 @@ void Session::set_sources(const float* table, int nsrc, uint64_t per_source) {
     msrc_n_ = per_source;
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
+    dep_cache_ = true;
 }
 
 void Session::set_source(const float pos[3], const float dir[3]) {
@@ -777,6 +797,7 @@ void Session::set_source(const float pos[3], const float dir[3]) {
     }
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
+    dep_cache_ = true;
 }
 
 RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n, float* gmua,

@@ -20,6 +20,7 @@ struct LaunchConfig {
     int tally_mode = -1;     // -1 tuned, 0 global atomics, 1 LDS-cached tile
     int vote = -1;           // scatter-round threshold in lanes (0: scatter right away, -1: tuned)
     int label_cache = 0;     // 0: labels in LDS when they fit, -1: labels read from HBM
+    int deposit_cache = 0;   // 0: chosen from the pilot, 1: always, -1: never
     bool trace = false;      // keep each photon's final state (parity tests; small runs only)
 };
 
@@ -79,6 +80,7 @@ public:
     int last_blocks() const { return last_blocks_; }
     int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
     int lds_label_bits() const { return (int)plabel_bits_; }  // 0 when labels are read from HBM
+    bool deposit_cache() const { return lc_.deposit_cache > 0 || (lc_.deposit_cache == 0 && dep_cache_); }
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
     float* device_fluence() const { return d_fluence_; }
@@ -112,6 +114,7 @@ private:
     // LDS tally tile {x0, y0, z0, g0, sx, sy, sz, sg}; sx = 0 means none
     int tile_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool tile_chosen_ = false;
+    bool dep_cache_ = true;              // per-lane deposit cache on (off when the tile takes most deposits)
     uint64_t tile_budget_ = 0;
     float* d_msrc_ = nullptr;            // multi-source table {pos xyz, dir xyz} per source
     uint64_t msrc_n_ = 0;                // photons per source (0: one source)
