@@ -347,6 +347,7 @@ def other_configs_leg(sync) -> dict:
             esc, tot, dep = sim.energies()
             out[name] = {"photons": n, "kernel_ms": round(kms, 3), "photons_per_ms": round(n / kms, 1),
                          "lds_label_bits": sim.info["lds_label_bits"], "tally_box": sim.tally_box(),
+                         "deposit_cache": sim.deposit_cache(), "deposit_cache_pilot_hits": round(sim.deposit_cache_hits(), 4),
                          "absorbed_percent": (tot - esc) / tot * 100.0}
             if name == "cube60_published_1e8":
                 # webpage/umcx.html:451,455 of the reference: 175,075 photons/ms on an RTX 5090
@@ -405,6 +406,7 @@ def main():
     esc, tot, dep = sim.energies()
     detected, saved = sim.detected()
     sim_box = sim.tally_box()
+    sim_cache = (sim.deposit_cache(), sim.deposit_cache_hits())
     sim.close()
     ms_per_step = wall_ms / args.steps
     value = args.photons * args.steps / wall_ms
@@ -432,6 +434,7 @@ def main():
                                  "peak_per_s": ATOMIC_PEAK,
                                  "frac": round(ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3) / ATOMIC_PEAK, 4)}},
         "tally_box": sim_box,
+        "deposit_cache": {"on": sim_cache[0], "pilot_hits": round(sim_cache[1], 4)},
         "detected_per_photon": detected / args.photons,
         "absorbed_percent": (tot - esc) / tot * 100.0,
     }

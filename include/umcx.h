@@ -64,6 +64,7 @@ typedef struct umcx_info {
                                          shape, runs under 12 photons per lane of the full shape use
                                          half the lanes */
     int32_t deposit_cache;            /* 1 when runs use the per-lane deposit cache */
+    double deposit_cache_hits;        /* share of the pilot's global deposits that hit that cache */
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)

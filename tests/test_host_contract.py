@@ -11,6 +11,7 @@
 * failures come back as error codes with the reference's messages, never as exceptions;
 * without a GPU the simulation path fails loudly instead of falling back to the CPU.
 """
+import ctypes as C
 import json
 import os
 import re
@@ -48,6 +49,26 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_native.SIGNATURES) - {cxx}
     nm = subprocess.run(["nm", "-D", "--defined-only", _native.LIB_PATH], capture_output=True, text=True).stdout
     assert cxx in nm     # C++ linkage entry of src/umcx.cpp:783
+
+
+def test_binding_structs_match_the_header(tmp_path):
+    """The ctypes mirrors of the C structs have the header's sizes and field offsets."""
+    checks = {"umcx_launch": _native.UmcxLaunch, "umcx_info": _native.UmcxInfo, "umcx_batch_trial": _native.UmcxBatchTrial}
+    src = ["#include <stddef.h>", "#include <stdio.h>", f'#include "{os.path.join(ROOT, "include", "umcx.h")}"',
+           "int main(void) {"]
+    for cname, py in checks.items():
+        src.append(f'    printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'    printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    src += ["    return 0;", "}"]
+    (tmp_path / "layout.c").write_text("\n".join(src))
+    subprocess.run(["gcc", "-o", str(tmp_path / "layout"), str(tmp_path / "layout.c")], check=True)
+    out = subprocess.run([str(tmp_path / "layout")], capture_output=True, text=True, check=True).stdout.split()
+    got = dict(zip(out[::2], (int(v) for v in out[1::2])))
+    for cname, py in checks.items():
+        assert got[cname] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[f"{cname}.{f}"] == getattr(py, f).offset, f"{cname}.{f}"
 
 
 @pytest.mark.parametrize("name", GOLDEN)

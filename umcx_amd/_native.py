@@ -26,7 +26,8 @@ class UmcxInfo(C.Structure):
                 ("srctype", C.c_int32), ("tstart", C.c_float), ("tend", C.c_float), ("rtstep", C.c_float),
                 ("unitinmm", C.c_float), ("seeds", C.c_uint32 * 4), ("block", C.c_int32), ("blocks", C.c_int32),
                 ("tally_mode", C.c_int32), ("label_bytes", C.c_int32), ("lds_label_bits", C.c_int32),
-                ("last_block", C.c_int32), ("last_blocks", C.c_int32), ("deposit_cache", C.c_int32)]
+                ("last_block", C.c_int32), ("last_blocks", C.c_int32), ("deposit_cache", C.c_int32),
+                ("deposit_cache_hits", C.c_double)]
 
 
 class UmcxBatchTrial(C.Structure):

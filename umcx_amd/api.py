@@ -105,6 +105,12 @@ class Simulation:
         check(lib().umcx_session_info(self._h, C.byref(info)))
         return bool(info.deposit_cache)
 
+    def deposit_cache_hits(self) -> float:
+        """Share of the pilot's global deposits that hit the per-lane deposit cache (0 before a pilot)."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return float(info.deposit_cache_hits)
+
     def run(self, first: int = 0, count: int | None = None) -> float:
         """Run photons [first, first+count); returns the kernel time in ms."""
         n = self.info["nphoton"] if count is None else int(count)

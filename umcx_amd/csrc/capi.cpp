@@ -199,6 +199,7 @@ int umcx_session_info(const umcx_session* s, umcx_info* info) {
         info->lds_label_bits = s->s->lds_label_bits();
         info->last_block = s->s->last_block(); info->last_blocks = s->s->last_blocks();
         info->deposit_cache = s->s->deposit_cache() ? 1 : 0;
+        info->deposit_cache_hits = s->s->deposit_cache_hits();
         return 0;
     });
 }

@@ -335,18 +335,19 @@ __device__ __forceinline__ bool boundary(Photon& p, float n1, float n2, Rng& r, 
 // Per-lane write-back cache of the last K global deposits, most recent first. A photon that
 // steps back into a voxel it left a moment ago (same time gate) adds to the cached partial sum
 // instead of issuing another memory-side atomic; an entry reaches the grid when it is evicted or
-// flushed. Only the float summation order of a cell changes.
+// flushed. Only the float summation order of a cell changes. Returns true on a miss (a new entry).
 constexpr uint64_t kNoKey = ~0ull;
 __device__ __forceinline__ uint64_t in_reg64(uint64_t x) {
     __asm__ volatile("" : "+v"(x));
     return x;
 }
 template <int K>
-__device__ __forceinline__ void cached_deposit(float* grid, uint64_t (&key)[K], float (&sum)[K], uint64_t idx,
+__device__ __forceinline__ bool cached_deposit(float* grid, uint64_t (&key)[K], float (&sum)[K], uint64_t idx,
                                                float val) {
     int pos = K;
 #pragma unroll
     for (int j = K - 1; j >= 0; j--) pos = key[j] == idx ? j : pos;
+    const bool miss = pos == K;
     float cur = 0.f;
 #pragma unroll
     for (int j = 0; j < K; j++) cur = pos == j ? in_reg(sum[j]) : cur;
@@ -362,6 +363,7 @@ __device__ __forceinline__ void cached_deposit(float* grid, uint64_t (&key)[K], 
     }
     key[0] = idx;
     sum[0] = cur + val;
+    return miss;
 }
 template <int K>
 __device__ __forceinline__ void flush_deposits(float* grid, uint64_t (&key)[K], float (&sum)[K]) {

@@ -42,6 +42,7 @@ constexpr int kRegPaths = 4;                       // partial paths kept in regi
 constexpr uint64_t kShortRunPhotonsPerLane = 12;
 constexpr int kDepCache = 2;                       // global deposits cached per lane
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
+constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
 
 namespace dev {
 
@@ -65,6 +66,7 @@ struct KernelArgs {
     double* energy;
     unsigned long long* cursor;
     TraceRec* trace;
+    unsigned long long* dep_stats;   // pilot only: {global deposits, deposit-cache misses}
     // LDS tally tile: box [tx0,tx0+tsx) x [ty0,..) x [tz0,..) over gates [tg0,tg0+tsg)
     int32_t tx0, ty0, tz0, tg0;
     uint32_t tsx, tsy, tsz, tsg, tcells, tile_n;
@@ -180,6 +182,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         dkey[j] = kNoKey;
         dsum[j] = 0.f;
     }
+    uint32_t nglob = 0, nmiss = 0;   // deposit-cache statistics (reported by the pilot)
     const bool energy_out = a.outputtype == kEnergy;
 
     while (true) {
@@ -299,10 +302,12 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                         if (p.tvox >= 0 && g < a.tsg)
                             __hip_atomic_fetch_add(stile + g * a.tcells + (unsigned)p.tvox, val, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
-                        else if (a.dep_cache)
-                            cached_deposit(a.fluence, dkey, dsum, (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
-                        else
+                        else if (a.dep_cache) {
+                            nglob++;
+                            nmiss += cached_deposit(a.fluence, dkey, dsum, (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
+                        } else {
                             unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
+                        }
                     }
                     p.lw = p.pw;
                     p.lz = 0.f;
@@ -383,6 +388,13 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     }
 
     if (savevol) flush_deposits(a.fluence, dkey, dsum);
+    if (a.dep_stats) {
+        const double g = wave_sum((double)nglob), m = wave_sum((double)nmiss);
+        if (__lane_id() == 0) {
+            atomicAdd(a.dep_stats, (unsigned long long)g);
+            atomicAdd(a.dep_stats + 1, (unsigned long long)m);
+        }
+    }
 
     // ---- flush the LDS tally tile ----
     if (a.tile_n) {
@@ -670,27 +682,32 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     uint64_t pilot = std::min<uint64_t>(count, 262144);
     float* d_cnt = dalloc<float>(fluence_len_);
     double* d_e = dalloc<double>(4);
-    unsigned long long* d_c = dalloc<unsigned long long>(2);
+    unsigned long long* d_c = dalloc<unsigned long long>(4);   // work cursor, deposit-cache statistics
     HIP_OK(hipMemsetAsync(d_cnt, 0, fluence_len_ * 4, s));
     HIP_OK(hipMemsetAsync(d_e, 0, 32, s));
-    HIP_OK(hipMemsetAsync(d_c, 0, 16, s));
+    HIP_OK(hipMemsetAsync(d_c, 0, 32, s));
     dev::KernelArgs ka;
     build_args(first, pilot, &ka);
     ka.fluence = d_cnt; ka.energy = d_e; ka.cursor = d_c; ka.trace = nullptr;
     ka.detnum = 0; ka.ppathlen = 0; ka.count_mode = 1;
+    ka.dep_cache = 1; ka.dep_stats = d_c + 2;
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     const bool reflect = p.isreflect != 0;
     pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
     std::vector<float> cnt(fluence_len_);
+    unsigned long long stats[2] = {0, 0};
     HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipMemcpyAsync(stats, d_c + 2, 16, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
+    dep_hits_ = stats[0] ? 1.0 - (double)stats[1] / (double)stats[0] : 0.0;
     (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
     const TallyBox box = choose_box(cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate, tile_budget_);
     for (int i = 0; i < 8; i++) tile_[i] = box[i];
-    // the per-lane deposit cache pays where most deposits go to HBM; when the tile takes most of
-    // them its extra work per deposit costs more than the atomics it saves
-    dep_cache_ = box_share(CountGrid{cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate}, box) < kCacheBelowShare;
+    // the per-lane deposit cache pays where most deposits go to HBM and a fair share of them hit
+    // it (the pilot ran with it); otherwise its extra work per deposit costs more than it saves
+    dep_cache_ = box_share(CountGrid{cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate}, box) < kCacheBelowShare &&
+                 dep_hits_ >= kCacheMinHits;
 }
 
 // Launch shape of one run. The full shape fills every CU; a run of fewer than

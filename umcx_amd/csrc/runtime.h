@@ -81,6 +81,7 @@ public:
     int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
     int lds_label_bits() const { return (int)plabel_bits_; }  // 0 when labels are read from HBM
     bool deposit_cache() const { return lc_.deposit_cache > 0 || (lc_.deposit_cache == 0 && dep_cache_); }
+    double deposit_cache_hits() const { return dep_hits_; }   // from the last pilot (0 before one)
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
     float* device_fluence() const { return d_fluence_; }
@@ -115,6 +116,7 @@ private:
     int tile_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     bool tile_chosen_ = false;
     bool dep_cache_ = true;              // per-lane deposit cache on (off when the tile takes most deposits)
+    double dep_hits_ = 0.0;              // share of the pilot's global deposits that hit the deposit cache
     uint64_t tile_budget_ = 0;
     float* d_msrc_ = nullptr;            // multi-source table {pos xyz, dir xyz} per source
     uint64_t msrc_n_ = 0;                // photons per source (0: one source)
