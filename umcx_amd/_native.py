@@ -83,6 +83,7 @@ SIGNATURES = {
     "umcx_seed_words": (C.c_int, [C.c_int32, C.POINTER(C.c_uint32)]),
     "umcx_libm_host": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64]),
     "umcx_libm_device": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32]),
+    "umcx_arith_check_device": (C.c_int, [C.c_int32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int32]),
     "umcx_free": (None, [C.c_void_p]),
 }
 

@@ -149,6 +149,49 @@ UMCX_HD void sincosf_host(float y, float* s, float* c) {
 UMCX_HD float sinf_host(float y) { float s, c; sincosf_host(y, &s, &c); return s; }
 UMCX_HD float cosf_host(float y) { float s, c; sincosf_host(y, &s, &c); return c; }
 
+UMCX_HD float sqrtf_cr(float x) {
+#if defined(__HIPCC__)
+    return __builtin_sqrtf(x);      // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
+#else
+    return sqrtf(x);
+#endif
+}
+
+// Correctly rounded sqrt and division for operands away from the special ranges. On the GPU
+// these are the compiler's correctly rounded sequences without their operand scaling and
+// special-case fix-up, which change nothing in these ranges:
+//  * sqrt_ns(x): x finite and >= 2^-96 (the scaling threshold; zero, inf and NaN excluded): the
+//    hardware estimate, then the neighbour below or above when its residual says so;
+//  * div_ns(a, b): 2^-120 <= |b| <= 2^120, and a == +0 or (2^-100 <= |a| and 2^-120 <= |a/b| <=
+//    2^120) (no v_div_scale case applies and v_div_fixup passes the quotient through):
+//    reciprocal estimate, one Newton step, quotient and two residual corrections.
+// Bitwise equal to sqrtf_cr and a / b there (exhaustive and sampled GPU checks in the tests).
+UMCX_HD float sqrt_ns(float x) {
+#if defined(__HIPCC__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float below = u2f(f2u(s) - 1u), above = u2f(f2u(s) + 1u);
+    const float t = __builtin_fmaf(-below, s, x) <= 0.f ? below : s;
+    return __builtin_fmaf(-above, s, x) > 0.f ? above : t;
+#else
+    return sqrtf(x);
+#endif
+}
+UMCX_HD float div_ns(float a, float b) {
+#if defined(__HIPCC__)
+    float r = __builtin_amdgcn_rcpf(b);
+    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    const float q0 = a * r;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r, q0);
+    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), r, q1);
+#else
+    return a / b;
+#endif
+}
+// 1 / x for a direction component (|x| <= 1 up to rounding): the short sequence for
+// 2^-60 <= |x| <= 2^60, the full correctly rounded division otherwise (zero, tiny or huge x).
+UMCX_HD float recip_dir(float x) {
+    return (fabsf(x) >= 0x1p-60f && fabsf(x) <= 0x1p60f) ? div_ns(1.0f, x) : 1.0f / x;
+}
 // ---- acosf: fdlibm single precision ----
 UMCX_HD float acos_rational(float z) {
     const float pS0 = 1.6666667163e-01f, pS1 = -3.2556581497e-01f, pS2 = 2.0121252537e-01f,
@@ -157,14 +200,9 @@ UMCX_HD float acos_rational(float z) {
                 qS4 = 7.7038154006e-02f;
     float p = z * (pS0 + z * (pS1 + z * (pS2 + z * (pS3 + z * (pS4 + z * pS5)))));
     float q = 1.0f + z * (qS1 + z * (qS2 + z * (qS3 + z * qS4)));
-    return p / q;
-}
-UMCX_HD float sqrtf_cr(float x) {
-#if defined(__HIPCC__)
-    return __builtin_sqrtf(x);      // correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt
-#else
-    return sqrtf(x);
-#endif
+    // q in [0.22, 1] and p near z / 6: the short division for z >= 2^-80 (acosf's small-argument
+    // range reaches z = 2^-114)
+    return z >= 0x1p-80f ? div_ns(p, q) : p / q;
 }
 UMCX_HD float acosf_host(float x) {
     const float pi = 3.1415925026e+00f, pio2_hi = 1.5707962513e+00f, pio2_lo = 7.5497894159e-08f;
@@ -183,13 +221,13 @@ UMCX_HD float acosf_host(float x) {
         if (ix <= 0x23000000) return pio2_hi + pio2_lo;
         return pio2_hi - (x - (pio2_lo - x * r));
     }
-    const float s = sqrtf_cr(z);
+    const float s = sqrt_ns(z);                 // z in [2^-25, 0.25]
     if (neg) {
         const float w = r * s - pio2_lo;
         return pi - 2.0f * (s + w);
     }
     const float df = u2f(f2u(s) & 0xfffff000u);
-    const float c = (z - df * df) / (s + df);
+    const float c = div_ns(z - df * df, s + df);   // s + df in [2^-13, 1]; numerator +0 or >= 2^-50
     const float w = r * s + c;
     return 2.0f * (df + w);
 }

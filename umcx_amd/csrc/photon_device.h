@@ -122,7 +122,7 @@ __device__ __forceinline__ void rotate(Photon& p, float st, float ct, float sp, 
     float nx, ny, nz;
     if (p.vz > -1.f + kEps && p.vz < 1.f - kEps) {
         float t0 = 1.f - p.vz * p.vz;
-        float t1 = st / libm::sqrtf_cr(t0);
+        float t1 = libm::div_ns(st, libm::sqrt_ns(t0));   // t0 >= 2^-23; st = sin of [0, pi], +0 or >= 2^-24
         nx = t1 * (p.vx * p.vz * cp - p.vy * sp) + p.vx * ct;
         ny = t1 * (p.vy * p.vz * cp + p.vx * sp) + p.vy * ct;
         nz = -t1 * t0 * cp + p.vz * ct;
@@ -131,7 +131,7 @@ __device__ __forceinline__ void rotate(Photon& p, float st, float ct, float sp, 
         ny = st * sp;
         nz = (p.vz > 0.f) ? ct : -ct;
     }
-    float s = 1.f / libm::sqrtf_cr(nx * nx + ny * ny + nz * nz);
+    float s = libm::div_ns(1.f, libm::sqrt_ns(nx * nx + ny * ny + nz * nz));   // a near-unit vector
     p.vx = nx * s;
     p.vy = ny * s;
     p.vz = nz * s;
@@ -258,17 +258,18 @@ __device__ __forceinline__ void scatter(Photon& p, Rng& r, float4 prop, const Ta
     libm::sincosf_host((2.f * kPi) * rng_u01(r), &sphi, &cphi);
     float th;
     if (fabsf(g) > kEps) {
-        float t = (1.f - g * g) / (1.f - g + 2.f * g * rng_u01(r));
-        t = (1.f + g * g - t * t) / (2.f * g);
+        // |g| in (2^-23, 1]: denominators in [2^-23, 2], numerators +0 or at least 2^-46
+        float t = libm::div_ns(1.f - g * g, 1.f - g + 2.f * g * rng_u01(r));
+        t = libm::div_ns(1.f + g * g - t * t, 2.f * g);
         th = libm::acosf_host(fmaxf(-1.f, fminf(1.f, t)));
     } else {
         th = libm::acosf_host(2.f * rng_u01(r) - 1.f);
     }
     libm::sincosf_host(th, &sth, &cth);
     rotate(p, sth, cth, sphi, cphi);
-    p.rx = 1.f / p.vx;
-    p.ry = 1.f / p.vy;
-    p.rz = 1.f / p.vz;
+    p.rx = libm::recip_dir(p.vx);
+    p.ry = libm::recip_dir(p.vy);
+    p.rz = libm::recip_dir(p.vz);
     p.vw += 1.f;
 }
 

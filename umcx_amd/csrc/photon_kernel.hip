@@ -434,6 +434,48 @@ __global__ void vprops_kernel(const float* mua, const float* mus, float unit, fl
     if (i < n) out[i] = make_float2(mua[i] * unit, mus[i] * unit);
 }
 
+// Self-check of the short correctly rounded sequences against the compiler's full ones, over a
+// whole input range, counting the inputs where they differ:
+//   0: sqrt_ns(x) vs sqrtf_cr(x) for every float in [2^-96, 2^128)
+//   1: recip_dir(x) vs 1/x for every float with 2^-60 <= |x| <= 2^60, both signs
+//   2: div_ns(a, b) vs a/b on 2^32 hashed pairs inside div_ns's domain
+__device__ __forceinline__ uint32_t arith_hash(uint64_t k) {
+    uint64_t z = k + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+}
+__global__ void arith_check_kernel(int which, uint64_t n, unsigned long long* bad) {
+    unsigned long long nbad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        bool same;
+        if (which == 0) {
+            const float x = libm::u2f(0x0F800000u + (uint32_t)i);
+            same = libm::f2u(libm::sqrt_ns(x)) == libm::f2u(libm::sqrtf_cr(x));
+        } else if (which == 1) {
+            const uint32_t span = (uint32_t)(n / 2);
+            const uint32_t mag = 0x21800000u + (uint32_t)(i % span);     // 2^-60 .. 2^60
+            const float x = libm::u2f(mag | (i >= span ? 0x80000000u : 0u));
+            same = libm::f2u(libm::recip_dir(x)) == libm::f2u(1.0f / x);
+        } else {
+            // |b| in [2^-120, 2^120], |a/b| in [2^-120, 2^120], |a| >= 2^-100, random signs and mantissas
+            const uint32_t h0 = arith_hash(2 * i), h1 = arith_hash(2 * i + 1);
+            const int eb = (int)(h0 % 241u) - 120;
+            int eq = (int)((h0 >> 8) % 241u) - 120;
+            int ea = eb + eq;
+            if (ea < -100) ea = -100;
+            if (ea > 127) ea = 127;
+            const float b = libm::u2f(((uint32_t)(eb + 127) << 23) | (h1 & 0x7FFFFFu) | ((h0 >> 31) << 31));
+            const float a = libm::u2f(((uint32_t)(ea + 127) << 23) | (arith_hash(i ^ 0x5555u) & 0x7FFFFFu) |
+                                      (((h0 >> 30) & 1u) << 31));
+            const float q = a / b;
+            same = !(fabsf(q) >= 0x1p-120f && fabsf(q) <= 0x1p120f) || libm::f2u(libm::div_ns(a, b)) == libm::f2u(q);
+        }
+        nbad += same ? 0ull : 1ull;
+    }
+    if (nbad) atomicAdd(bad, nbad);
+}
+
 __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -882,6 +924,21 @@ void Session::fetch_trace(TraceRec* out, uint64_t n) const {
     if (!d_trace_) throw std::runtime_error("trace was not enabled for this session");
     n = std::min(n, last_count_);
     HIP_OK(hipMemcpy(out, d_trace_, n * sizeof(TraceRec), hipMemcpyDeviceToHost));
+}
+
+uint64_t device_arith_check(int which, uint64_t* tested, int device) {
+    HIP_OK(hipSetDevice(device));
+    const uint64_t n = which == 0 ? (0x7F800000ull - 0x0F800000ull)
+                     : which == 1 ? 2ull * (0x5D800001ull - 0x21800000ull) : (1ull << 32);
+    unsigned long long* bad = dalloc<unsigned long long>(1);
+    HIP_OK(hipMemset(bad, 0, 8));
+    hipLaunchKernelGGL(dev::arith_check_kernel, dim3(8192), dim3(256), 0, 0, which, n, bad);
+    HIP_OK(hipGetLastError());
+    unsigned long long h = 0;
+    HIP_OK(hipMemcpy(&h, bad, 8, hipMemcpyDeviceToHost));
+    (void)hipFree(bad);
+    if (tested) *tested = n;
+    return h;
 }
 
 void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device) {

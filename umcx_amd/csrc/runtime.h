@@ -136,6 +136,9 @@ public:
 // Replica checks: evaluate the device libm replicas on n inputs (fn: 0 logf, 1 sinf, 2 cosf,
 // 3 acosf, 4 expf) on the GPU.
 void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device);
+// Self-check of the short correctly rounded sqrt, reciprocal and division sequences against the
+// full ones (which: 0 sqrt, 1 reciprocal, 2 division); returns the number of differing inputs.
+uint64_t device_arith_check(int which, uint64_t* tested, int device);
 
 int device_count();
 std::string device_name(int device);
