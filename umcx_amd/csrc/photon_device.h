@@ -296,6 +296,10 @@ __device__ __forceinline__ float in_reg(float x) {
     __asm__ volatile("" : "+v"(x));
     return x;
 }
+__device__ __forceinline__ uint32_t in_reg_u32(uint32_t x) {
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
 __device__ __forceinline__ float pick_axis(bool ax, bool ay, float x, float y, float z) {
     return ax ? in_reg(x) : (ay ? in_reg(y) : in_reg(z));
 }
@@ -337,13 +341,11 @@ __device__ __forceinline__ bool boundary(Photon& p, float n1, float n2, Rng& r, 
 // steps back into a voxel it left a moment ago (same time gate) adds to the cached partial sum
 // instead of issuing another memory-side atomic; an entry reaches the grid when it is evicted or
 // flushed. Only the float summation order of a cell changes. Returns true on a miss (a new entry).
-constexpr uint64_t kNoKey = ~0ull;
-__device__ __forceinline__ uint64_t in_reg64(uint64_t x) {
-    __asm__ volatile("" : "+v"(x));
-    return x;
-}
+// Keys are 32-bit fluence indices (grids of fewer than 2^32 - 1 cells; the all-ones key marks an
+// empty entry).
+constexpr uint32_t kNoKey = ~0u;
 template <int K>
-__device__ __forceinline__ bool cached_deposit(float* grid, uint64_t (&key)[K], float (&sum)[K], uint64_t idx,
+__device__ __forceinline__ bool cached_deposit(float* grid, uint32_t (&key)[K], float (&sum)[K], uint32_t idx,
                                                float val) {
     int pos = K;
 #pragma unroll
@@ -352,14 +354,14 @@ __device__ __forceinline__ bool cached_deposit(float* grid, uint64_t (&key)[K], 
     float cur = 0.f;
 #pragma unroll
     for (int j = 0; j < K; j++) cur = pos == j ? in_reg(sum[j]) : cur;
-    if (pos == K) {                         // miss: the least recent entry goes to the grid
+    if (miss) {                             // the least recent entry goes to the grid
         if (key[K - 1] != kNoKey) unsafeAtomicAdd(grid + key[K - 1], sum[K - 1]);
         pos = K - 1;
     }
 #pragma unroll
     for (int j = K - 1; j >= 1; j--) {      // entries above pos move down one place
         const bool sh = j <= pos;
-        key[j] = sh ? in_reg64(key[j - 1]) : in_reg64(key[j]);
+        key[j] = sh ? in_reg_u32(key[j - 1]) : in_reg_u32(key[j]);
         sum[j] = sh ? in_reg(sum[j - 1]) : in_reg(sum[j]);
     }
     key[0] = idx;
@@ -367,7 +369,7 @@ __device__ __forceinline__ bool cached_deposit(float* grid, uint64_t (&key)[K], 
     return miss;
 }
 template <int K>
-__device__ __forceinline__ void flush_deposits(float* grid, uint64_t (&key)[K], float (&sum)[K]) {
+__device__ __forceinline__ void flush_deposits(float* grid, uint32_t (&key)[K], float (&sum)[K]) {
 #pragma unroll
     for (int j = 0; j < K; j++) {
         if (key[j] != kNoKey) unsafeAtomicAdd(grid + key[j], sum[j]);

@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     float pp0 = 0.f, pp1 = 0.f, pp2 = 0.f, pp3 = 0.f;
     uint64_t rbase = 0, rk = 0;
     const bool savevol = !REPLAY && a.issavevol != 0;
-    uint64_t dkey[kDepCache];
+    uint32_t dkey[kDepCache];
     float dsum[kDepCache];
 #pragma unroll
     for (int j = 0; j < kDepCache; j++) {
@@ -304,7 +304,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                         else if (a.dep_cache) {
                             nglob++;
-                            nmiss += cached_deposit(a.fluence, dkey, dsum, (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
+                            nmiss += cached_deposit(a.fluence, dkey, dsum, (uint32_t)p.lastvox + (uint32_t)tshift * (uint32_t)a.nxyz, val);
                         } else {
                             unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
                         }

@@ -80,7 +80,10 @@ public:
     int last_blocks() const { return last_blocks_; }
     int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
     int lds_label_bits() const { return (int)plabel_bits_; }  // 0 when labels are read from HBM
-    bool deposit_cache() const { return lc_.deposit_cache > 0 || (lc_.deposit_cache == 0 && dep_cache_); }
+    // the cache keys are 32-bit fluence indices: grids of 2^32 - 1 cells or more never use it
+    bool deposit_cache() const {
+        return fluence_len_ < 0xFFFFFFFFull && (lc_.deposit_cache > 0 || (lc_.deposit_cache == 0 && dep_cache_));
+    }
     double deposit_cache_hits() const { return dep_hits_; }   // from the last pilot (0 before one)
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
