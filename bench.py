@@ -408,6 +408,16 @@ def main():
     sim_box = sim.tally_box()
     sim_cache = (sim.deposit_cache(), sim.deposit_cache_hits())
     sim.close()
+    # the drop-in path end to end (outside the timed steps): a fresh session uploads the labels,
+    # runs the pilot and the photons, and reads the fluence and the detected rows back to the host
+    sync()
+    t1 = time.perf_counter()
+    with U.Simulation(cfg) as e2e:
+        e2e.run()
+        e2e_flu = e2e.fluence()
+        e2e_rows = e2e.detp()[0]
+    e2e_ms = (time.perf_counter() - t1) * 1e3
+    log(f"bench: end to end {e2e_ms:.1f} ms ({e2e_flu.nbytes / 1e6:.1f} MB fluence, {len(e2e_rows)} rows read back)")
     ms_per_step = wall_ms / args.steps
     value = args.photons * args.steps / wall_ms
     kavg = sum(kernel_ms) / len(kernel_ms)
@@ -433,6 +443,9 @@ def main():
                                  "achieved_per_s": ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3),
                                  "peak_per_s": ATOMIC_PEAK,
                                  "frac": round(ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3) / ATOMIC_PEAK, 4)}},
+        "end_to_end": {"wall_ms": round(e2e_ms, 1), "photons_per_ms": round(args.photons / e2e_ms, 1),
+                       "includes": "session set-up, label upload, counting pilot, photon loop, read-back of the "
+                                   "fluence grid and the detected rows over PCIe"},
         "tally_box": sim_box,
         "deposit_cache": {"on": sim_cache[0], "pilot_hits": round(sim_cache[1], 4)},
         "detected_per_photon": detected / args.photons,
