@@ -1,16 +1,21 @@
 // photon_kernel.hip -- gfx950 photon-transport kernel (plain form) and its Session runtime.
 //
-// One lane owns one photon at a time. Waves are persistent: a lane whose photon ends takes the
-// next photon index from a device cursor (one atomic per wave per refill), so lanes stay busy
-// while their neighbours' photons live on. Photon i is a pure function of (seed words, i,
-// problem) -- its xorshift128+ stream is reseeded from i (src/umcx.cpp:761) -- so the order in
-// which lanes take photons does not change any photon's trajectory.
+// One lane owns one photon at a time. Waves are persistent: the lanes of a wave whose photons
+// ended take new photon slots together -- one LDS atomic on the workgroup's static share, or one
+// device atomic on the shared tail of the range -- so lanes stay busy while their neighbours'
+// photons live on. Photon i is a pure function of (seed words, i, problem) -- its xorshift128+
+// stream is reseeded from i (src/umcx.cpp:761) -- so the order in which lanes take photons does
+// not change any photon's trajectory.
 //
 // Arithmetic follows the reference's FP32 operation order exactly (no contraction: built with
 // -ffp-contract=off; division and sqrt correctly rounded; denormals kept), and the
 // libm calls use the bit-exact replicas of glibc_libm.h -- logf/sinf/cosf/acosf steer the photon;
 // expf (src/umcx.cpp:326) only decays its weight, and is replicated too so that weights and
 // fluence deposits are also bitwise equal to the reference's, up to atomic summation order.
+//
+// Fluence deposits go to an LDS tally box (chosen from a counting pilot), else through a
+// per-lane cache of the last two global deposits, else straight to HBM as float atomics, which
+// execute at the memory side and are the kernel's binding ceiling.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
