@@ -5,7 +5,7 @@
 //   atomics: device-scope float atomic adds (no return) to hashed addresses in a 40 MiB buffer
 //            (the size of a time-gated fluence grid), in atomics per second, and the same into
 //            one hot 256-float region (same-address serialisation at the memory side), and the
-//            hashed rate over footprints from 64 KiB to 1 GiB.
+//            hashed rate over footprints from 64 KiB to 1 GiB, and with the nt / sc1 cache-policy bits.
 //
 // Prints one JSON line. Every index is taken modulo the buffer size; the launches are bounded.
 #include <hip/hip_runtime.h>
@@ -45,6 +45,18 @@ __device__ __forceinline__ uint32_t mix(uint32_t x) {
 __global__ void __launch_bounds__(256) spread_atomics(float* buf, uint32_t n, uint32_t iters) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t k = 0; k < iters; k++) unsafeAtomicAdd(buf + mix(tid * 2654435761u + k) % n, 1.0f);
+}
+
+// the same hashed atomics with the non-temporal or system-scope cache-policy bits (no return value)
+template <int POLICY>
+__global__ void __launch_bounds__(256) spread_atomics_policy(float* buf, uint32_t n, uint32_t iters) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t k = 0; k < iters; k++) {
+        float* p = buf + mix(tid * 2654435761u + k) % n;
+        const float one = 1.0f;
+        if (POLICY == 1) asm volatile("global_atomic_add_f32 %0, %1, off nt" : : "v"(p), "v"(one) : "memory");
+        else asm volatile("global_atomic_add_f32 %0, %1, off sc1" : : "v"(p), "v"(one) : "memory");
+    }
 }
 
 int main() {
@@ -97,6 +109,13 @@ int main() {
            "\"spread_atomics_per_s\": %.4e, \"hot_atomics_per_s\": %.4e}\n",
            prop.gcnArchName, prop.multiProcessorCount, winstr / (fms * 1e-3), fms, natom / (sms * 1e-3),
            natom / (hms * 1e-3));
+    auto nt = [&] { hipLaunchKernelGGL(spread_atomics_policy<1>, dim3(ablocks), dim3(athreads), 0, 0, buf, n, aiters); };
+    auto sc1 = [&] { hipLaunchKernelGGL(spread_atomics_policy<2>, dim3(ablocks), dim3(athreads), 0, 0, buf, n, aiters); };
+    time_ms(nt);
+    const double nt_rate = natom / (time_ms(nt) * 1e-3);
+    time_ms(sc1);
+    const double sc1_rate = natom / (time_ms(sc1) * 1e-3);
+    printf("{\"spread_atomics_nt_per_s\": %.4e, \"spread_atomics_sc1_per_s\": %.4e}\n", nt_rate, sc1_rate);
     printf("{\"spread_atomics_per_s_by_footprint_mib\": {");
     for (size_t i = 0; i < sizeof fps / sizeof fps[0]; i++)
         printf("%s\"%g\": %.4e", i ? ", " : "", fps[i] * 4.0 / (1 << 20), fp_rate[i]);
