@@ -149,6 +149,30 @@ def test_four_bit_lds_labels_match_oracle():
     assert np.array_equal(rows.view(np.uint32), sorted_rows(o["detp"]).view(np.uint32))
 
 
+def test_unusual_refractive_indices_match_oracle():
+    """Layers with refractive indices far from tissue values (0.45 and 4.5 beside 1.37): total
+    internal reflection at most angles and large transmission bends; every photon matches the
+    oracle."""
+    cfg = layered_config()
+    cfg["Session"]["ID"] = "layers_n"
+    cfg["Domain"]["Media"][2]["n"] = 4.5
+    cfg["Domain"]["Media"][4]["n"] = 0.45
+    prob = O.problem_from_config(cfg, U.domain(cfg))
+    n = cfg["Session"]["Photons"]
+    say("trace parity with refractive indices 0.45 and 4.5")
+    o = O.run(prob, nphoton=n, traces=True, nthreads=8)
+    with U.Simulation(cfg, trace=True) as sim:
+        sim.run(0, n)
+        t = sim.trace(n)
+        rows = sorted_rows(sim.detp()[0])
+    ot = o["traces"]
+    exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
+    for key in ("pos", "vec", "len"):
+        assert np.array_equal(exact(t[key]), exact(ot[key])), key
+    assert np.array_equal(t["mediaid"], ot["mediaid"])
+    assert np.array_equal(rows.view(np.uint32), sorted_rows(o["detp"]).view(np.uint32))
+
+
 def test_detector_overflow_keeps_genuine_rows():
     g, prob = O.golden_problem("overflow")
     with U.Simulation(g["cfg"]) as sim:
