@@ -182,14 +182,15 @@ def run(prob: dict, nphoton: int | None = None, first: int = 0, traces: bool = F
     flu = np.zeros(prob["maxgate"] * nx * ny * nz, dtype=np.float32) if prob["issavevol"] else None
     detlen = L.oracle_detp_len(C.byref(op))
     maxdet = prob["maxdetphotons"] if prob["issavedet"] else 0
-    cap = min(maxdet, max(n, 1))
+    # a photon is saved once per detector whose disk holds its exit point (detectors may overlap)
+    cap = min(maxdet, max(n, 1) * max(len(detpos), 1))
     detp = np.zeros((cap, max(detlen, 1)), dtype=np.float32)
     detid = np.zeros(cap, dtype=np.uint32)
     detw = np.zeros(cap, dtype=np.float32)
     ndet = C.c_uint32(0)
     en = (C.c_double * 3)()
     tr = np.zeros(n, dtype=TRACE_DTYPE) if traces else None
-    # the oracle writes at most maxdetphotons rows; size the buffer by the photon count instead
+    # the oracle writes at most maxdetphotons rows; size the buffer by photons x detectors instead
     saved_max = op.maxdetphotons
     op.maxdetphotons = cap
     rc = L.oracle_run(C.byref(op), first, n, flu.ctypes.data if flu is not None else None,

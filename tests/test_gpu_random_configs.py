@@ -1,4 +1,4 @@
-"""Randomised GPU parity: seeded random configurations (volume size, label layout with background
+"""Randomised GPU parity: 300 seeded random configurations (volume size, label layout with background
 holes, 1-7 media with and without refractive-index steps, zero-scattering media, every source type
 inside and outside the volume, reflection on/off, 1-5 time gates, detectors with every saved
 column, all output types, length units) run on the GPU through the C ABI and on the oracle; the
@@ -74,7 +74,8 @@ def random_config(seed: int) -> dict:
     return cfg
 
 
-@pytest.mark.parametrize("seed", list(range(100)))
+# seed 782: overlapping detectors save more rows (1,625) than there are photons (1,500)
+@pytest.mark.parametrize("seed", list(range(300)) + [782])
 def test_random_config_matches_oracle(seed):
     cfg = random_config(seed)
     src = cfg["Optode"]["Source"]
