@@ -129,7 +129,7 @@ int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int3
 /* The LDS tally tile in use, {x0, y0, z0, gate0, nx, ny, nz, ngates} (zeros when none).
  * No counterpart in the reference. */
 int umcx_session_tally_box(const umcx_session* s, int32_t box[8]);
-/* Output normaliser of src/umcx.cpp:820 for a given launched energy. */
+/* Output normaliser of src/umcx.cpp:820 for a given launched energy (NaN for a null session). */
 float umcx_session_normalizer(const umcx_session* s, double energytotal);
 void umcx_session_free(umcx_session* s);
 

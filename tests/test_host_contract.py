@@ -188,6 +188,45 @@ def test_batch_rejects_an_empty_job_list():
     assert "at least one job" in U.last_error()
 
 
+def test_null_handles_return_error_codes():
+    """Every session and batch entry point turns a null handle into an error code and a message
+    (no exception crosses the C ABI, nothing is dereferenced); the checks run before any GPU call."""
+    import math
+    from umcx_amd._native import UmcxInfo, lib
+    L = lib()
+    nul = C.c_void_p()
+    f = (C.c_float * 4)()
+    d = (C.c_double * 3)()
+    i32 = (C.c_int32 * 8)()
+    calls = {
+        "umcx_session_info": lambda: L.umcx_session_info(nul, C.byref(UmcxInfo())),
+        "umcx_session_reset": lambda: L.umcx_session_reset(nul),
+        "umcx_session_run": lambda: L.umcx_session_run(nul, 0, 1, None),
+        "umcx_session_fluence": lambda: L.umcx_session_fluence(nul, f, 4),
+        "umcx_session_detected": lambda: L.umcx_session_detected(nul, None, None),
+        "umcx_session_detp": lambda: L.umcx_session_detp(nul, None, None, None),
+        "umcx_session_energies": lambda: L.umcx_session_energies(nul, d),
+        "umcx_session_tally_box": lambda: L.umcx_session_tally_box(nul, i32),
+        "umcx_session_set_source": lambda: L.umcx_session_set_source(nul, f, f),
+        "umcx_batch_run": lambda: L.umcx_batch_run(nul, None),
+        "umcx_batch_info": lambda: L.umcx_batch_info(nul, 0, C.byref(UmcxInfo())),
+        "umcx_batch_energies": lambda: L.umcx_batch_energies(nul, 0, d),
+        "umcx_batch_fluence": lambda: L.umcx_batch_fluence(nul, 0, f, 4),
+        "umcx_batch_tally_box": lambda: L.umcx_batch_tally_box(nul, 0, i32, None),
+        "umcx_batch_trials": lambda: L.umcx_batch_trials(nul, None, 0, None),
+    }
+    for name, call in calls.items():
+        assert call() == 1, name
+        assert U.last_error().startswith(name + ": null"), (name, U.last_error())
+    assert math.isnan(L.umcx_session_normalizer(nul, 1.0))
+    assert "null session" in U.last_error()
+    assert L.umcx_seed_words(1, None) == 1
+    assert L.umcx_libm_host(0, None, None, 4) == 1
+    assert L.umcx_libm_host(0, None, None, 0) == 0
+    L.umcx_session_free(nul)
+    L.umcx_batch_free(nul)
+
+
 def test_reference_structural_cli_checks():
     """The structural checks of the reference's test/testumcx.sh:13-53 against this build's
     binary: it exists and is executable, its libraries resolve, the help lists at least five long

@@ -3,11 +3,13 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <limits>
 #include <memory>
 #include <string>
 
@@ -59,6 +61,17 @@ umcx::LaunchConfig to_launch(const umcx_launch* l) {
     return lc;
 }
 
+// handles passed in by C callers: a null or freed-looking handle is an error, not a crash
+umcx::Session& session_of(const umcx_session* s, const char* fn) {
+    if (!s || !s->s) throw std::runtime_error(std::string(fn) + ": null session");
+    return *s->s;
+}
+
+umcx::BatchSession& batch_of(const umcx_batch* b, const char* fn) {
+    if (!b || !b->b) throw std::runtime_error(std::string(fn) + ": null batch");
+    return *b->b;
+}
+
 // Normaliser of src/umcx.cpp:820: double arithmetic, stored as float
 float normalizer_of(const umcx::Problem& p, double energytotal) {
     if (p.outputtype == umcx::kEnergy) return (float)(1.f / energytotal);
@@ -69,6 +82,19 @@ float normalizer_of(const umcx::Problem& p, double energytotal) {
 
 std::string out_name(const umcx::Problem& p, const char* suffix, const char* dflt) {
     return p.session_id.empty() ? std::string(dflt) : p.session_id + suffix;
+}
+
+void fill_info(const umcx::Problem& p, umcx_info* info) {
+    memset(info, 0, sizeof *info);
+    info->dims[0] = p.nx; info->dims[1] = p.ny; info->dims[2] = p.nz; info->dims[3] = (uint32_t)p.maxgate;
+    info->nphoton = p.nphoton;
+    info->mediumnum = p.mediumnum; info->detnum = p.detnum;
+    info->detp_columns = p.detp_columns(); info->ppath_columns = p.ppath_columns();
+    info->outputtype = p.outputtype; info->isnormalized = p.isnormalized; info->issavevol = p.issavevol;
+    info->issavedet = p.issavedet; info->isreflect = p.isreflect; info->srctype = p.srctype;
+    info->tstart = p.tstart; info->tend = p.tend; info->rtstep = p.rtstep; info->unitinmm = p.unitinmm;
+    for (int i = 0; i < 4; i++) info->seeds[i] = p.seeds[i];
+    info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
 }
 
 }  // namespace
@@ -183,34 +209,25 @@ int umcx_session_from_argv(char** argv, int argn, const umcx_launch* launch, umc
 
 int umcx_session_info(const umcx_session* s, umcx_info* info) {
     return guarded([&]() -> int {
-        if (!s || !info) throw std::runtime_error("umcx_session_info: null argument");
-        const umcx::Problem& p = s->s->problem();
-        memset(info, 0, sizeof *info);
-        info->dims[0] = p.nx; info->dims[1] = p.ny; info->dims[2] = p.nz; info->dims[3] = (uint32_t)p.maxgate;
-        info->nphoton = p.nphoton;
-        info->mediumnum = p.mediumnum; info->detnum = p.detnum;
-        info->detp_columns = p.detp_columns(); info->ppath_columns = p.ppath_columns();
-        info->outputtype = p.outputtype; info->isnormalized = p.isnormalized; info->issavevol = p.issavevol;
-        info->issavedet = p.issavedet; info->isreflect = p.isreflect; info->srctype = p.srctype;
-        info->tstart = p.tstart; info->tend = p.tend; info->rtstep = p.rtstep; info->unitinmm = p.unitinmm;
-        for (int i = 0; i < 4; i++) info->seeds[i] = p.seeds[i];
-        info->block = s->s->block(); info->blocks = s->s->blocks(); info->tally_mode = s->s->tally_mode();
-        info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
-        info->lds_label_bits = s->s->lds_label_bits();
-        info->last_block = s->s->last_block(); info->last_blocks = s->s->last_blocks();
-        info->deposit_cache = s->s->deposit_cache() ? 1 : 0;
-        info->deposit_cache_hits = s->s->deposit_cache_hits();
+        const umcx::Session& ss = session_of(s, "umcx_session_info");
+        if (!info) throw std::runtime_error("umcx_session_info: null argument");
+        fill_info(ss.problem(), info);
+        info->block = ss.block(); info->blocks = ss.blocks(); info->tally_mode = ss.tally_mode();
+        info->lds_label_bits = ss.lds_label_bits();
+        info->last_block = ss.last_block(); info->last_blocks = ss.last_blocks();
+        info->deposit_cache = ss.deposit_cache() ? 1 : 0;
+        info->deposit_cache_hits = ss.deposit_cache_hits();
         return 0;
     });
 }
 
 int umcx_session_reset(umcx_session* s) {
-    return guarded([&]() -> int { s->s->reset(); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_reset").reset(); return 0; });
 }
 
 int umcx_session_run(umcx_session* s, uint64_t first, uint64_t count, double* kernel_ms) {
     return guarded([&]() -> int {
-        umcx::RunStats st = s->s->run(first, count);
+        umcx::RunStats st = session_of(s, "umcx_session_run").run(first, count);
         if (kernel_ms) *kernel_ms = st.kernel_ms;
         return 0;
     });
@@ -218,50 +235,51 @@ int umcx_session_run(umcx_session* s, uint64_t first, uint64_t count, double* ke
 
 int umcx_session_fluence(const umcx_session* s, float* out, uint64_t n) {
     return guarded([&]() -> int {
-        if (n < s->s->fluence_len()) throw std::runtime_error("umcx_session_fluence: buffer too small");
-        s->s->fetch_fluence(out);
+        const umcx::Session& ss = session_of(s, "umcx_session_fluence");
+        if (!out || n < ss.fluence_len()) throw std::runtime_error("umcx_session_fluence: buffer too small");
+        ss.fetch_fluence(out);
         return 0;
     });
 }
 
 int umcx_session_fluence_device(const umcx_session* s, float** dptr) {
-    return guarded([&]() -> int { *dptr = s->s->device_fluence(); return 0; });
+    return guarded([&]() -> int { *dptr = session_of(s, "umcx_session_fluence_device").device_fluence(); return 0; });
 }
 
 int umcx_session_detected(const umcx_session* s, uint32_t* detected, uint32_t* saved) {
     return guarded([&]() -> int {
-        uint32_t d = s->s->detected();
-        if (detected) *detected = d;
-        if (saved) *saved = s->s->saved();
+        const umcx::Session& ss = session_of(s, "umcx_session_detected");
+        if (detected) *detected = ss.detected();
+        if (saved) *saved = ss.saved();
         return 0;
     });
 }
 
 int umcx_session_detp(const umcx_session* s, float* rows, uint32_t* ids, float* weights) {
-    return guarded([&]() -> int { s->s->fetch_detp(rows, ids, weights); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_detp").fetch_detp(rows, ids, weights); return 0; });
 }
 
 int umcx_session_energies(const umcx_session* s, double out[3]) {
-    return guarded([&]() -> int { s->s->energies(out); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_energies").energies(out); return 0; });
 }
 
 int umcx_session_trace(const umcx_session* s, void* out, uint64_t n) {
-    return guarded([&]() -> int { s->s->fetch_trace((umcx::TraceRec*)out, n); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_trace").fetch_trace((umcx::TraceRec*)out, n); return 0; });
 }
 
 int umcx_session_set_voxel_props(umcx_session* s, const float* mua, const float* mus, int32_t on_device) {
-    return guarded([&]() -> int { s->s->set_voxel_props(mua, mus, on_device != 0); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_set_voxel_props").set_voxel_props(mua, mus, on_device != 0); return 0; });
 }
 
 int umcx_session_set_source(umcx_session* s, const float pos[3], const float dir[3]) {
-    return guarded([&]() -> int { s->s->set_source(pos, dir); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_set_source").set_source(pos, dir); return 0; });
 }
 
 int umcx_session_detp_device(const umcx_session* s, uint32_t** ids, float** weights, float** rows) {
     return guarded([&]() -> int {
-        if (ids) *ids = s->s->device_detid();
-        if (weights) *weights = s->s->device_detw();
-        if (rows) *rows = s->s->device_detp();
+        if (ids) *ids = session_of(s, "umcx_session_detp_device").device_detid();
+        if (weights) *weights = session_of(s, "umcx_session_detp_device").device_detw();
+        if (rows) *rows = session_of(s, "umcx_session_detp_device").device_detp();
         return 0;
     });
 }
@@ -269,14 +287,14 @@ int umcx_session_detp_device(const umcx_session* s, uint32_t** ids, float** weig
 int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n,
                         float* grad_mua, float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms) {
     return guarded([&]() -> int {
-        umcx::RunStats st = s->s->replay(ids, slices, coef, n, grad_mua, grad_mus, slice_stride, final_w);
+        umcx::RunStats st = session_of(s, "umcx_session_replay").replay(ids, slices, coef, n, grad_mua, grad_mus, slice_stride, final_w);
         if (kernel_ms) *kernel_ms = st.kernel_ms;
         return 0;
     });
 }
 
 int umcx_session_set_sources(umcx_session* s, const float* sources, int32_t nsrc, uint64_t photons_per_source) {
-    return guarded([&]() -> int { s->s->set_sources(sources, nsrc, photons_per_source); return 0; });
+    return guarded([&]() -> int { session_of(s, "umcx_session_set_sources").set_sources(sources, nsrc, photons_per_source); return 0; });
 }
 
 int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int32_t* slices, const float* coef,
@@ -284,7 +302,7 @@ int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int3
                                 float* grad_mus, uint64_t slice_stride, float* final_w, double* kernel_ms) {
     return guarded([&]() -> int {
         if (!src_index || !sources) throw std::runtime_error("umcx_session_replay_sources: null source arrays");
-        umcx::RunStats st = s->s->replay(ids, slices, coef, n, grad_mua, grad_mus, slice_stride, final_w, src_index,
+        umcx::RunStats st = session_of(s, "umcx_session_replay_sources").replay(ids, slices, coef, n, grad_mua, grad_mus, slice_stride, final_w, src_index,
                                          sources);
         if (kernel_ms) *kernel_ms = st.kernel_ms;
         return 0;
@@ -293,29 +311,18 @@ int umcx_session_replay_sources(umcx_session* s, const uint32_t* ids, const int3
 
 int umcx_session_tally_box(const umcx_session* s, int32_t box[8]) {
     return guarded([&]() -> int {
-        for (int i = 0; i < 8; i++) box[i] = s->s->tile()[i];
+        for (int i = 0; i < 8; i++) box[i] = session_of(s, "umcx_session_tally_box").tile()[i];
         return 0;
     });
 }
 
 float umcx_session_normalizer(const umcx_session* s, double energytotal) {
-    return normalizer_of(s->s->problem(), energytotal);
+    float v = std::numeric_limits<float>::quiet_NaN();
+    guarded([&]() -> int { v = normalizer_of(session_of(s, "umcx_session_normalizer").problem(), energytotal); return 0; });
+    return v;
 }
 
 void umcx_session_free(umcx_session* s) { delete s; }
-
-static void fill_info(const umcx::Problem& p, umcx_info* info) {
-    memset(info, 0, sizeof *info);
-    info->dims[0] = p.nx; info->dims[1] = p.ny; info->dims[2] = p.nz; info->dims[3] = (uint32_t)p.maxgate;
-    info->nphoton = p.nphoton;
-    info->mediumnum = p.mediumnum; info->detnum = p.detnum;
-    info->detp_columns = p.detp_columns(); info->ppath_columns = p.ppath_columns();
-    info->outputtype = p.outputtype; info->isnormalized = p.isnormalized; info->issavevol = p.issavevol;
-    info->issavedet = p.issavedet; info->isreflect = p.isreflect; info->srctype = p.srctype;
-    info->tstart = p.tstart; info->tend = p.tend; info->rtstep = p.rtstep; info->unitinmm = p.unitinmm;
-    for (int i = 0; i < 4; i++) info->seeds[i] = p.seeds[i];
-    info->label_bytes = p.mediumnum <= 128 ? 1 : 4;
-}
 
 int umcx_batch_create(const char* const* jsons, int32_t njobs, const umcx_launch* launch, umcx_batch** out) {
     return guarded([&]() -> int {
@@ -337,7 +344,7 @@ int umcx_batch_create(const char* const* jsons, int32_t njobs, const umcx_launch
 
 int umcx_batch_run(umcx_batch* b, double* kernel_ms) {
     return guarded([&]() -> int {
-        umcx::RunStats st = b->b->run();
+        umcx::RunStats st = batch_of(b, "umcx_batch_run").run();
         if (kernel_ms) *kernel_ms = st.kernel_ms;
         return 0;
     });
@@ -345,36 +352,39 @@ int umcx_batch_run(umcx_batch* b, double* kernel_ms) {
 
 int umcx_batch_info(const umcx_batch* b, int32_t job, umcx_info* info) {
     return guarded([&]() -> int {
-        fill_info(b->b->problem(job), info);
-        info->block = b->b->block();
-        info->blocks = b->b->blocks();
-        info->tally_mode = b->b->tile_cells() > 0 ? 1 : 0;
+        const umcx::BatchSession& bs = batch_of(b, "umcx_batch_info");
+        if (!info) throw std::runtime_error("umcx_batch_info: null argument");
+        fill_info(bs.problem(job), info);
+        info->block = bs.block();
+        info->blocks = bs.blocks();
+        info->tally_mode = bs.tile_cells() > 0 ? 1 : 0;
         return 0;
     });
 }
 
 int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n) {
     return guarded([&]() -> int {
-        if (n < b->b->fluence_len(job)) throw std::runtime_error("umcx_batch_fluence: buffer too small");
-        b->b->fetch_fluence(job, out);
+        const umcx::BatchSession& bs = batch_of(b, "umcx_batch_fluence");
+        if (!out || n < bs.fluence_len(job)) throw std::runtime_error("umcx_batch_fluence: buffer too small");
+        bs.fetch_fluence(job, out);
         return 0;
     });
 }
 
 int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]) {
-    return guarded([&]() -> int { b->b->energies(job, out); return 0; });
+    return guarded([&]() -> int { batch_of(b, "umcx_batch_energies").energies(job, out); return 0; });
 }
 
 int umcx_batch_trials(const umcx_batch* b, umcx_batch_trial* out, int32_t cap, int32_t* n) {
     return guarded([&]() -> int {
-        const auto& t = b->b->trials();
+        const umcx::BatchSession& bs = batch_of(b, "umcx_batch_trials");
+        const auto& t = bs.trials();
         if (n) *n = (int32_t)t.size();
         for (int32_t i = 0; out && i < cap && i < (int32_t)t.size(); i++) {
             out[i].block = t[i].block;
             out[i].blocks = t[i].blocks;
             out[i].tile_cells = t[i].tile_cells;
-            out[i].chosen = t[i].block == b->b->block() && t[i].blocks == b->b->blocks() &&
-                            t[i].tile_cells == b->b->tile_cells();
+            out[i].chosen = t[i].block == bs.block() && t[i].blocks == bs.blocks() && t[i].tile_cells == bs.tile_cells();
             out[i].ms = t[i].ms;
             out[i].tile_share = t[i].tile_share;
         }
@@ -384,9 +394,11 @@ int umcx_batch_trials(const umcx_batch* b, umcx_batch_trial* out, int32_t cap, i
 
 int umcx_batch_tally_box(const umcx_batch* b, int32_t job, int32_t box[8], double* deposits_per_photon) {
     return guarded([&]() -> int {
-        const umcx::TallyBox& bx = b->b->box(job);
+        const umcx::BatchSession& bs = batch_of(b, "umcx_batch_tally_box");
+        if (!box) throw std::runtime_error("umcx_batch_tally_box: null argument");
+        const umcx::TallyBox& bx = bs.box(job);
         for (int i = 0; i < 8; i++) box[i] = bx[i];
-        if (deposits_per_photon) *deposits_per_photon = b->b->cost(job);
+        if (deposits_per_photon) *deposits_per_photon = bs.cost(job);
         return 0;
     });
 }
@@ -395,9 +407,10 @@ void umcx_batch_free(umcx_batch* b) { delete b; }
 
 int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels) {
     return guarded([&]() -> int {
+        if (!json || !dims || !labels) throw std::runtime_error("umcx_domain_from_json: null argument");
         std::vector<int32_t> lab;
         umcx::build_domain(umcx::Json::parse(json), dims[0], dims[1], dims[2], lab);
-        *labels = (int32_t*)malloc(lab.size() * 4);
+        *labels = (int32_t*)malloc(std::max<size_t>(lab.size(), 1) * 4);
         if (!*labels) throw std::runtime_error("out of host memory");
         memcpy(*labels, lab.data(), lab.size() * 4);
         return 0;
@@ -408,7 +421,9 @@ int umcx_config_from_argv(char** argv, int argn, char** json_out) {
     return guarded([&]() -> int {
         umcx::ParsedInput in = umcx::parse_argv(argn, argv);
         std::string s = in.cfg.dump(2);
+        if (!json_out) throw std::runtime_error("umcx_config_from_argv: null argument");
         *json_out = (char*)malloc(s.size() + 1);
+        if (!*json_out) throw std::runtime_error("out of host memory");
         memcpy(*json_out, s.c_str(), s.size() + 1);
         return 0;
     });
@@ -426,12 +441,20 @@ int umcx_choose_tally_box(const float* counts, int32_t nx, int32_t ny, int32_t n
 }
 
 int umcx_seed_words(int32_t seed, uint32_t out[4]) {
+    if (!out) {
+        g_last_error = "umcx_seed_words: null argument";
+        return 1;
+    }
     umcx::glibc_seed_words(seed, out);
     return 0;
 }
 
 int umcx_libm_host(int32_t fn, const float* in, float* out, uint64_t n) {
     namespace L = umcx::libm;
+    if (n && (!in || !out)) {
+        g_last_error = "umcx_libm_host: null argument";
+        return 1;
+    }
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; i++) {
         float x = in[i];
