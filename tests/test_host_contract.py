@@ -153,6 +153,10 @@ def test_error_paths_return_codes(tmp_path):
         assert "type must be number" in U.last_error()
         assert U.run_json("{not json") == 1
         assert "parse_error" in U.last_error()
+        huge = U.config_from_argv(["--bench", "cube60"])
+        huge["Domain"]["Dim"] = [32767, 32767, 3]       # 3.2e9 voxels: rejected before allocation
+        assert U.run_json(huge) == 1
+        assert U.last_error() == "Domain must hold fewer than 2^31 voxels"
     finally:
         os.chdir(cwd)
 

@@ -347,6 +347,8 @@ void build_domain(const Json& cfg, uint32_t& nx, uint32_t& ny, uint32_t& nz, std
         nz = (uint32_t)size.at(2).as_uint();
         if (nx == 0 || ny == 0 || nz == 0 || nx > 32767 || ny > 32767 || nz > 32767)
             fail("Domain dimensions must be within 1..32767 voxels per axis");
+        // voxel indices are 32-bit signed integers, as the reference's index() returns (:128-130)
+        if ((uint64_t)nx * ny * nz > 0x7fffffffull) fail("Domain must hold fewer than 2^31 voxels");
         vol.assign((size_t)nx * ny * nz, value);
     };
     reshape(dom.at("Dim"), 0);
