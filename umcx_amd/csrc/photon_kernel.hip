@@ -48,6 +48,7 @@ constexpr uint64_t kShortRunPhotonsPerLane = 12;
 constexpr int kDepCache = 2;                       // global deposits cached per lane
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
+constexpr uint64_t kPilotHostCells = 1ull << 26;   // pilot count grids above this reach the host as a window
 
 namespace dev {
 
@@ -433,6 +434,39 @@ __global__ void zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// Pilot count grids too large to copy to the host: the largest count (counts are >= 0, so their
+// bit patterns order like their values), the first cell holding it, the total, and a window
+// [t][x][y][z] around that cell packed into a contiguous buffer.
+__global__ void count_max_kernel(const float* c, uint64_t n, unsigned* maxbits) {
+    unsigned m = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        m = max(m, __float_as_uint(c[i]));
+    if (m) atomicMax(maxbits, m);
+}
+
+__global__ void count_argmax_sum_kernel(const float* c, uint64_t n, const unsigned* maxbits,
+                                        unsigned long long* first, double* total) {
+    const unsigned m = *maxbits;
+    unsigned long long f = ~0ull;
+    double t = 0.0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const float v = c[i];
+        t += (double)v;
+        if (__float_as_uint(v) == m && i < f) f = i;
+    }
+    if (f != ~0ull) atomicMin(first, f);
+    if (t != 0.0) unsafeAtomicAdd(total, t);
+}
+
+__global__ void count_window_kernel(const float* c, float* out, int t0, int x0, int y0, int z0, int wt, int wx, int wy,
+                                    int wz, uint64_t nxyz, uint64_t nyz, int nz) {
+    const uint64_t wyz = (uint64_t)wy * wz, wxyz = wyz * wx, n = wxyz * wt;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t t = i / wxyz, r = i % wxyz, x = r / wyz, yz = r % wyz, y = yz / wz, z = yz % wz;
+        out[i] = c[(t0 + t) * nxyz + (x0 + x) * nyz + (y0 + y) * (uint64_t)nz + (z0 + z)];
+    }
+}
+
 // per-voxel {mua, mus} in 1/mm -> per-voxel-length units, as the label media are scaled (:804)
 __global__ void vprops_kernel(const float* mua, const float* mus, float unit, float2* out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -721,40 +755,87 @@ size_t Session::lds_bytes(uint32_t tile_n, bool paths_needed) const {
 }
 
 // Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
+// Count grids of up to kPilotHostCells cells are searched whole on the host; larger ones through
+// a window of at most that many cells centred on the densest cell (halving the longest extent
+// until it fits), with the total taken on the device. The pilot's deposits go through the
+// per-lane cache only where its 32-bit keys address every cell.
 void Session::choose_tile(uint64_t first, uint64_t count) {
     tile_chosen_ = true;
     const Problem& p = prob_;
     if (!p.issavevol || tally_mode_ == 0 || tile_budget_ < 64) return;
     hipStream_t s = (hipStream_t)stream_;
     uint64_t pilot = std::min<uint64_t>(count, 262144);
-    float* d_cnt = dalloc<float>(fluence_len_);
+    const uint64_t n = fluence_len_;
+    const bool cache = n < 0xFFFFFFFFull;
+    float* d_cnt = dalloc<float>(n);
     double* d_e = dalloc<double>(4);
-    unsigned long long* d_c = dalloc<unsigned long long>(4);   // work cursor, deposit-cache statistics
-    HIP_OK(hipMemsetAsync(d_cnt, 0, fluence_len_ * 4, s));
+    unsigned long long* d_c = dalloc<unsigned long long>(8);   // cursor, cache statistics, argmax, max bits, total
+    HIP_OK(hipMemsetAsync(d_cnt, 0, n * 4, s));
     HIP_OK(hipMemsetAsync(d_e, 0, 32, s));
-    HIP_OK(hipMemsetAsync(d_c, 0, 32, s));
+    HIP_OK(hipMemsetAsync(d_c, 0, 64, s));
     dev::KernelArgs ka;
     build_args(first, pilot, &ka);
     ka.fluence = d_cnt; ka.energy = d_e; ka.cursor = d_c; ka.trace = nullptr;
     ka.detnum = 0; ka.ppathlen = 0; ka.count_mode = 1;
-    ka.dep_cache = 1; ka.dep_stats = d_c + 2;
+    ka.dep_cache = cache ? 1 : 0; ka.dep_stats = cache ? d_c + 2 : nullptr;
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     const bool reflect = p.isreflect != 0;
     pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
-    std::vector<float> cnt(fluence_len_);
+    const int ext[4] = {p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz};   // (t, x, y, z)
+    int off[4] = {0, 0, 0, 0}, win[4] = {ext[0], ext[1], ext[2], ext[3]};
+    std::vector<float> cnt;
+    double total = -1.0;
+    if (n > kPilotHostCells) {
+        unsigned* d_max = (unsigned*)(d_c + 5);
+        double* d_tot = (double*)(d_c + 6);
+        HIP_OK(hipMemsetAsync(d_c + 4, 0xFF, 8, s));                 // argmax starts at ~0
+        dev::count_max_kernel<<<2048, 256, 0, s>>>(d_cnt, n, d_max);
+        dev::count_argmax_sum_kernel<<<2048, 256, 0, s>>>(d_cnt, n, d_max, d_c + 4, d_tot);
+        HIP_OK(hipGetLastError());
+        unsigned long long best = 0;
+        HIP_OK(hipMemcpyAsync(&best, d_c + 4, 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(&total, d_tot, 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (best == ~0ull) best = 0;                                  // an empty count grid
+        const uint64_t nyz = (uint64_t)p.ny * p.nz, nxyz = nyz * p.nx;
+        const int at[4] = {(int)(best / nxyz), (int)(best % nxyz / nyz), (int)(best % nyz / p.nz), (int)(best % p.nz)};
+        auto cells = [&] { uint64_t c = 1; for (int d = 0; d < 4; d++) c *= (uint64_t)win[d]; return c; };
+        while (cells() > kPilotHostCells) {
+            int d = 0;
+            for (int k = 1; k < 4; k++) if (win[k] > win[d]) d = k;
+            win[d] = (win[d] + 1) / 2;
+        }
+        for (int d = 0; d < 4; d++) off[d] = std::min(std::max(at[d] - win[d] / 2, 0), ext[d] - win[d]);
+        float* d_win = dalloc<float>(cells());
+        dev::count_window_kernel<<<2048, 256, 0, s>>>(d_cnt, d_win, off[0], off[1], off[2], off[3], win[0], win[1], win[2],
+                                                  win[3], nxyz, nyz, (int)p.nz);
+        HIP_OK(hipGetLastError());
+        cnt.resize(cells());
+        HIP_OK(hipMemcpyAsync(cnt.data(), d_win, cells() * 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        (void)hipFree(d_win);
+    } else {
+        cnt.resize(n);
+        HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, n * 4, hipMemcpyDeviceToHost, s));
+    }
     unsigned long long stats[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, fluence_len_ * 4, hipMemcpyDeviceToHost, s));
     HIP_OK(hipMemcpyAsync(stats, d_c + 2, 16, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
     dep_hits_ = stats[0] ? 1.0 - (double)stats[1] / (double)stats[0] : 0.0;
     (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
-    const TallyBox box = choose_box(cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate, tile_budget_);
+    const CountGrid grid{cnt.data(), win[1], win[2], win[3], win[0]};
+    TallyBox box = choose_box(grid.cnt, grid.nx, grid.ny, grid.nz, grid.nt, tile_budget_);
+    const TallyBox whole{0, 0, 0, 0, win[1], win[2], win[3], win[0]};
+    const double in_box = box_count(grid, box);
+    if (total < 0) total = box_count(grid, whole);
+    if (box[4] > 0) {
+        box[0] += off[1]; box[1] += off[2]; box[2] += off[3]; box[3] += off[0];
+    }
     for (int i = 0; i < 8; i++) tile_[i] = box[i];
     // the per-lane deposit cache pays where most deposits go to HBM and a fair share of them hit
     // it (the pilot ran with it); otherwise its extra work per deposit costs more than it saves
-    dep_cache_ = box_share(CountGrid{cnt.data(), (int)p.nx, (int)p.ny, (int)p.nz, p.maxgate}, box) < kCacheBelowShare &&
-                 dep_hits_ >= kCacheMinHits;
+    dep_cache_ = (total > 0 ? in_box / total : 0.0) < kCacheBelowShare && dep_hits_ >= kCacheMinHits;
 }
 
 // Launch shape of one run. The full shape fills every CU; a run of fewer than

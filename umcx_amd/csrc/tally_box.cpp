@@ -91,14 +91,18 @@ std::vector<TallyBox> choose_boxes(const std::vector<CountGrid>& grids, uint64_t
     return out;
 }
 
-double box_share(const CountGrid& g, const TallyBox& b) {
+double box_count(const CountGrid& g, const TallyBox& b) {
+    if (b[4] == 0) return 0.0;
     const uint64_t nyz = (uint64_t)g.ny * g.nz, nxyz = nyz * g.nx;
     const View v{g.cnt, nyz, nxyz, g.nz};
-    const int all_l[4] = {0, 0, 0, 0}, all_h[4] = {g.nt, g.nx, g.ny, g.nz};
-    const double tot = box_sum(v, all_l, all_h);
-    if (!(tot > 0) || b[4] == 0) return 0.0;
     const int l[4] = {b[3], b[0], b[1], b[2]}, h[4] = {b[3] + b[7], b[0] + b[4], b[1] + b[5], b[2] + b[6]};
-    return box_sum(v, l, h) / tot;
+    return box_sum(v, l, h);
+}
+
+double box_share(const CountGrid& g, const TallyBox& b) {
+    const double tot = box_count(g, TallyBox{0, 0, 0, 0, g.nx, g.ny, g.nz, g.nt});
+    if (!(tot > 0) || b[4] == 0) return 0.0;
+    return box_count(g, b) / tot;
 }
 
 }  // namespace umcx

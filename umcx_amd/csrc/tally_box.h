@@ -24,7 +24,8 @@ struct CountGrid {
 // choose_box for many grids at once (OpenMP over grids)
 std::vector<TallyBox> choose_boxes(const std::vector<CountGrid>& grids, uint64_t budget);
 
-// fraction of a grid's counts that fall inside a box
+// counts inside a box, and the fraction of a grid's counts that fall inside it
+double box_count(const CountGrid& g, const TallyBox& b);
 double box_share(const CountGrid& g, const TallyBox& b);
 
 }  // namespace umcx
