@@ -128,6 +128,7 @@ CASES = {
     "gated": (["--bench", "spherebox", "--json", '{"Forward":{"T0":0,"T1":1e-9,"Dt":1e-10}}',
                "-n", "20000", "-w", "53", "-U", "0", "-O", "e"], None),
     "normx": (["--bench", "cubesph60b", "-n", "5000", "-d", "0", "-u", "0.5"], None),
+    "zerophotons": (["--bench", "cube60", "-n", "0"], None),
 }
 FULL_FLUENCE_MAX = 300_000   # store the whole volume when it has at most this many cells
 SAMPLE = 4096

@@ -248,6 +248,33 @@ def test_command_line_outputs_are_byte_compatible(name, tmp_path):
         assert np.array_equal(sorted_rows(rows).view(np.uint32), g["detp"].view(np.uint32))
 
 
+def test_zero_photons_write_the_reference_files(tmp_path, capfd):
+    """`-n 0`: the reference still writes both files. Its normaliser is 1/0 = inf, so every
+    fluence value is 0 * inf, a NaN with the host's bit pattern; the detected-photon file holds no
+    rows; the summary line reports zero energy and a NaN absorbed fraction."""
+    g = O.golden("zerophotons")
+    argv = list(g["argv"])
+    say("CLI outputs with zero photons")
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        assert U.run_cmd(argv) == 0, U.last_error()
+    finally:
+        os.chdir(cwd)
+    line = capfd.readouterr().out.strip().splitlines()[-1]
+    ref = str(g["line"])
+    cut = lambda s: (s.split(", duration")[0], s.split(" ms, ")[1])       # the duration differs
+    assert cut(line) == cut(ref), (line, ref)
+    raw = (tmp_path / "zerophotons.bnii").read_bytes()
+    assert len(raw) == int(g["bnii_size"]) and raw[:128] == bytes(g["bnii_head"])
+    vals = bjdata.load_bnii(str(tmp_path / "zerophotons.bnii"))
+    flat = vals.transpose(3, 0, 1, 2).reshape(-1)
+    assert np.isnan(flat).all()
+    assert np.array_equal(flat.view(np.uint32), g["fluence"].view(np.uint32))
+    detp = (tmp_path / "zerophotons_detp.jdb").read_bytes()
+    assert len(detp) == int(g["detp_size"]) and detp == bytes(g["detp_head"])[:len(detp)]
+
+
 def test_photon_ranges_compose():
     """Photon i depends only on i: two half runs give the same detections as one full run."""
     g = O.golden("c2")

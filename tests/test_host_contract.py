@@ -26,9 +26,10 @@ from umcx_amd import _native, bjdata
 
 ROOT = O.ROOT
 GOLDEN = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel", "c2",
-          "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx"]
+          "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx",
+          "zerophotons"]
 BENCH_ARGV = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel",
-              "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx"]
+              "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx", "zerophotons"]
 
 
 def header_functions():
