@@ -130,15 +130,18 @@ int MCX_run_simulation(char* argv[], int argn, int nlhs, void* plhs[]) {
         if (umcx::device_count() <= 0) throw std::runtime_error("no HIP device available: the umcx GPU path needs an MI355X (gfx950)");
         if (hipSetDevice(lc.device) != hipSuccess || hipFree(nullptr) != hipSuccess)
             throw std::runtime_error("cannot initialise HIP device " + std::to_string(lc.device));
+        // host outputs are allocated before the timer, as the reference allocates its output
+        // volume and detected-photon buffer (:797-799) before starting it (:812)
+        std::vector<float> flu(prob.issavevol ? (size_t)prob.nx * prob.ny * prob.nz * (size_t)prob.maxgate : 0);
+        std::vector<float> rows(prob.issavedet ? (size_t)prob.maxdetphotons * std::max(prob.detp_columns(), 1) : 0);
         // timing window of src/umcx.cpp:812-840: device upload, the photon loop, read-back
         auto t0 = std::chrono::steady_clock::now();
         umcx::Session sess(prob, lc);
         sess.run(0, prob.nphoton);
-        std::vector<float> flu(sess.fluence_len());
+        if (flu.size() != sess.fluence_len()) throw std::runtime_error("fluence size mismatch");
         sess.fetch_fluence(flu.data());
         uint32_t saved = sess.saved();
-        std::vector<float> rows((size_t)saved * std::max(prob.detp_columns(), 1));
-        sess.fetch_detp(rows.data(), nullptr, nullptr);
+        if (prob.issavedet) sess.fetch_detp(rows.data(), nullptr, nullptr);
         double e[3];
         sess.energies(e);
         double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

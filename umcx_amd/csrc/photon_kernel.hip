@@ -27,7 +27,6 @@
 #include "glibc_libm.h"
 #include "photon_device.h"
 #include "runtime.h"
-#include "tally_box.h"
 
 #define HIP_OK(x)                                                                                   \
     do {                                                                                            \
@@ -48,7 +47,6 @@ constexpr uint64_t kShortRunPhotonsPerLane = 12;
 constexpr int kDepCache = 2;                       // global deposits cached per lane
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
-constexpr uint64_t kPilotHostCells = 1ull << 26;   // pilot count grids above this reach the host as a window
 
 namespace dev {
 
@@ -434,9 +432,8 @@ __global__ void zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
-// Pilot count grids too large to copy to the host: the largest count (counts are >= 0, so their
-// bit patterns order like their values), the first cell holding it, the total, and a window
-// [t][x][y][z] around that cell packed into a contiguous buffer.
+// Pilot count grids stay on the device: the largest count (counts are >= 0, so their bit patterns
+// order like their values), the first cell holding it and the total.
 __global__ void count_max_kernel(const float* c, uint64_t n, unsigned* maxbits) {
     unsigned m = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
@@ -458,12 +455,100 @@ __global__ void count_argmax_sum_kernel(const float* c, uint64_t n, const unsign
     if (t != 0.0) unsafeAtomicAdd(total, t);
 }
 
-__global__ void count_window_kernel(const float* c, float* out, int t0, int x0, int y0, int z0, int wt, int wx, int wy,
-                                    int wz, uint64_t nxyz, uint64_t nyz, int nz) {
-    const uint64_t wyz = (uint64_t)wy * wz, wxyz = wyz * wx, n = wxyz * wt;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t t = i / wxyz, r = i % wxyz, x = r / wyz, yz = r % wyz, y = yz / wz, z = yz % wz;
-        out[i] = c[(t0 + t) * nxyz + (x0 + x) * nyz + (y0 + y) * (uint64_t)nz + (z0 + z)];
+// Sum of one value per thread over a workgroup of up to 1024 lanes; every thread gets the sum.
+__device__ double workgroup_sum(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < nw; k++) t += red[k];
+    return t;
+}
+
+// Counts in the box [l, h) of a [t][x][y][z] count grid, one workgroup.
+__device__ double box_sum_wg(const float* cnt, uint64_t nxyz, uint64_t nyz, int nz, const int* l, const int* h,
+                             double* red) {
+    const uint64_t L3 = h[3] - l[3], L2 = h[2] - l[2], L1 = h[1] - l[1], L0 = h[0] - l[0];
+    const uint64_t n = L0 * L1 * L2 * L3;
+    double s = 0.0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t z = i % L3, y = i / L3 % L2, x = i / (L3 * L2) % L1, t = i / (L3 * L2 * L1);
+        s += (double)cnt[(l[0] + t) * nxyz + (l[1] + x) * nyz + (l[2] + y) * (uint64_t)nz + (l[3] + z)];
+    }
+    return workgroup_sum(s, red);
+}
+
+// The greedy growth of choose_box (tally_box.cpp) over a count grid left in device memory: start
+// at the densest cell, then add the face slab with the most counts per added cell (candidates in
+// the order t, x, y, z, low side first; ties keep the first) while the box holds at most `budget`
+// cells. Counts are whole numbers, so every sum is exact and each choice is the host's.
+// out_box: {x0, y0, z0, g0, sx, sy, sz, sg} (zeros for an empty grid); out_in: counts inside.
+__global__ void __launch_bounds__(1024) grow_box_kernel(const float* cnt, int nt, int nx, int ny, int nz, uint64_t budget,
+                                                        const unsigned long long* start, int* out_box, double* out_in) {
+    __shared__ int lo[4], hi[4], go;
+    __shared__ double red[16];
+    const uint64_t nyz = (uint64_t)ny * nz, nxyz = nyz * nx;
+    const int ext[4] = {nt, nx, ny, nz};
+    if (threadIdx.x == 0) {
+        const unsigned long long b = *start;
+        go = (b != ~0ull && budget > 0 && cnt[b] > 0.f) ? 1 : 0;
+        if (go) {
+            lo[0] = (int)(b / nxyz);
+            lo[1] = (int)(b % nxyz / nyz);
+            lo[2] = (int)(b % nyz / nz);
+            lo[3] = (int)(b % nz);
+            for (int d = 0; d < 4; d++) hi[d] = lo[d] + 1;
+        }
+    }
+    __syncthreads();
+    if (!go) {
+        if (threadIdx.x < 8) out_box[threadIdx.x] = 0;
+        if (threadIdx.x == 0) *out_in = 0.0;
+        return;
+    }
+    while (true) {
+        int l0[4], h0[4];
+        for (int k = 0; k < 4; k++) { l0[k] = lo[k]; h0[k] = hi[k]; }
+        uint64_t cur = 1;
+        for (int k = 0; k < 4; k++) cur *= (uint64_t)(h0[k] - l0[k]);
+        int bd = -1, bside = 0;
+        double bscore = 0.0;
+        for (int c = 0; c < 8; c++) {
+            const int d = c >> 1, side = c & 1;
+            const int idx = side ? h0[d] : l0[d] - 1;
+            if (idx < 0 || idx >= ext[d]) continue;
+            const uint64_t nc = cur / (uint64_t)(h0[d] - l0[d]) * (uint64_t)(h0[d] - l0[d] + 1);
+            if (nc > budget) continue;
+            int l[4], h[4];
+            for (int k = 0; k < 4; k++) { l[k] = l0[k]; h[k] = h0[k]; }
+            l[d] = idx;
+            h[d] = idx + 1;
+            const double add = box_sum_wg(cnt, nxyz, nyz, nz, l, h, red);
+            const double score = add / (double)(nc - cur);
+            if (add > 0 && score > bscore) {
+                bscore = score;
+                bd = d;
+                bside = side;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (bd < 0) go = 0;
+            else if (bside) hi[bd]++;
+            else lo[bd]--;
+        }
+        __syncthreads();
+        if (!go) break;
+    }
+    int l[4], h[4];
+    for (int k = 0; k < 4; k++) { l[k] = lo[k]; h[k] = hi[k]; }
+    const double in = box_sum_wg(cnt, nxyz, nyz, nz, l, h, red);
+    if (threadIdx.x == 0) {
+        const int b[8] = {l[1], l[2], l[3], l[0], h[1] - l[1], h[2] - l[2], h[3] - l[3], h[0] - l[0]};
+        for (int k = 0; k < 8; k++) out_box[k] = b[k];
+        *out_in = in;
     }
 }
 
@@ -755,10 +840,10 @@ size_t Session::lds_bytes(uint32_t tile_n, bool paths_needed) const {
 }
 
 // Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
-// Count grids of up to kPilotHostCells cells are searched whole on the host; larger ones through
-// a window of at most that many cells centred on the densest cell (halving the longest extent
-// until it fits), with the total taken on the device. The pilot's deposits go through the
-// per-lane cache only where its 32-bit keys address every cell.
+// The count grid never leaves the device: two passes find the densest cell (the first of equal
+// ones) and the total, and one workgroup grows the box from that cell exactly as choose_box
+// does on the host. The pilot's deposits go through the per-lane cache only where its 32-bit
+// keys address every cell.
 void Session::choose_tile(uint64_t first, uint64_t count) {
     tile_chosen_ = true;
     const Problem& p = prob_;
@@ -769,10 +854,13 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     const bool cache = n < 0xFFFFFFFFull;
     float* d_cnt = dalloc<float>(n);
     double* d_e = dalloc<double>(4);
-    unsigned long long* d_c = dalloc<unsigned long long>(8);   // cursor, cache statistics, argmax, max bits, total
+    // 0 cursor, 2-3 cache statistics, 4 densest cell, 5 largest count bits, 6 total, 7 in box,
+    // 8-11 the box (8 ints)
+    unsigned long long* d_c = dalloc<unsigned long long>(12);
     HIP_OK(hipMemsetAsync(d_cnt, 0, n * 4, s));
     HIP_OK(hipMemsetAsync(d_e, 0, 32, s));
-    HIP_OK(hipMemsetAsync(d_c, 0, 64, s));
+    HIP_OK(hipMemsetAsync(d_c, 0, 96, s));
+    HIP_OK(hipMemsetAsync(d_c + 4, 0xFF, 8, s));                    // densest cell starts at ~0
     dev::KernelArgs ka;
     build_args(first, pilot, &ka);
     ka.fluence = d_cnt; ka.energy = d_e; ka.cursor = d_c; ka.trace = nullptr;
@@ -782,57 +870,20 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     const bool reflect = p.isreflect != 0;
     pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
-    const int ext[4] = {p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz};   // (t, x, y, z)
-    int off[4] = {0, 0, 0, 0}, win[4] = {ext[0], ext[1], ext[2], ext[3]};
-    std::vector<float> cnt;
-    double total = -1.0;
-    if (n > kPilotHostCells) {
-        unsigned* d_max = (unsigned*)(d_c + 5);
-        double* d_tot = (double*)(d_c + 6);
-        HIP_OK(hipMemsetAsync(d_c + 4, 0xFF, 8, s));                 // argmax starts at ~0
-        dev::count_max_kernel<<<2048, 256, 0, s>>>(d_cnt, n, d_max);
-        dev::count_argmax_sum_kernel<<<2048, 256, 0, s>>>(d_cnt, n, d_max, d_c + 4, d_tot);
-        HIP_OK(hipGetLastError());
-        unsigned long long best = 0;
-        HIP_OK(hipMemcpyAsync(&best, d_c + 4, 8, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(&total, d_tot, 8, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (best == ~0ull) best = 0;                                  // an empty count grid
-        const uint64_t nyz = (uint64_t)p.ny * p.nz, nxyz = nyz * p.nx;
-        const int at[4] = {(int)(best / nxyz), (int)(best % nxyz / nyz), (int)(best % nyz / p.nz), (int)(best % p.nz)};
-        auto cells = [&] { uint64_t c = 1; for (int d = 0; d < 4; d++) c *= (uint64_t)win[d]; return c; };
-        while (cells() > kPilotHostCells) {
-            int d = 0;
-            for (int k = 1; k < 4; k++) if (win[k] > win[d]) d = k;
-            win[d] = (win[d] + 1) / 2;
-        }
-        for (int d = 0; d < 4; d++) off[d] = std::min(std::max(at[d] - win[d] / 2, 0), ext[d] - win[d]);
-        float* d_win = dalloc<float>(cells());
-        dev::count_window_kernel<<<2048, 256, 0, s>>>(d_cnt, d_win, off[0], off[1], off[2], off[3], win[0], win[1], win[2],
-                                                  win[3], nxyz, nyz, (int)p.nz);
-        HIP_OK(hipGetLastError());
-        cnt.resize(cells());
-        HIP_OK(hipMemcpyAsync(cnt.data(), d_win, cells() * 4, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        (void)hipFree(d_win);
-    } else {
-        cnt.resize(n);
-        HIP_OK(hipMemcpyAsync(cnt.data(), d_cnt, n * 4, hipMemcpyDeviceToHost, s));
-    }
-    unsigned long long stats[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(stats, d_c + 2, 16, hipMemcpyDeviceToHost, s));
+    dev::count_max_kernel<<<2048, 256, 0, s>>>(d_cnt, n, (unsigned*)(d_c + 5));
+    dev::count_argmax_sum_kernel<<<2048, 256, 0, s>>>(d_cnt, n, (const unsigned*)(d_c + 5), d_c + 4, (double*)(d_c + 6));
+    dev::grow_box_kernel<<<1, 1024, 0, s>>>(d_cnt, p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz, tile_budget_, d_c + 4,
+                                            (int*)(d_c + 8), (double*)(d_c + 7));
+    HIP_OK(hipGetLastError());
+    unsigned long long head[12];
+    HIP_OK(hipMemcpyAsync(head, d_c, sizeof head, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    dep_hits_ = stats[0] ? 1.0 - (double)stats[1] / (double)stats[0] : 0.0;
     (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
-    const CountGrid grid{cnt.data(), win[1], win[2], win[3], win[0]};
-    TallyBox box = choose_box(grid.cnt, grid.nx, grid.ny, grid.nz, grid.nt, tile_budget_);
-    const TallyBox whole{0, 0, 0, 0, win[1], win[2], win[3], win[0]};
-    const double in_box = box_count(grid, box);
-    if (total < 0) total = box_count(grid, whole);
-    if (box[4] > 0) {
-        box[0] += off[1]; box[1] += off[2]; box[2] += off[3]; box[3] += off[0];
-    }
-    for (int i = 0; i < 8; i++) tile_[i] = box[i];
+    dep_hits_ = head[2] ? 1.0 - (double)head[3] / (double)head[2] : 0.0;
+    double total, in_box;
+    memcpy(&total, &head[6], 8);
+    memcpy(&in_box, &head[7], 8);
+    memcpy(tile_, &head[8], sizeof(int) * 8);
     // the per-lane deposit cache pays where most deposits go to HBM and a fair share of them hit
     // it (the pilot ran with it); otherwise its extra work per deposit costs more than it saves
     dep_cache_ = (total > 0 ? in_box / total : 0.0) < kCacheBelowShare && dep_hits_ >= kCacheMinHits;
