@@ -1,9 +1,9 @@
 """Fluence grids past 2^32 cells (HBM-sized outputs).
 
 A 1024^3-voxel cube with 5 time gates has 5.4e9 fluence cells (21.5 GB of float32): every deposit
-in gate 4 lands at cell 2^32 or beyond, and the pilot's count grid and the LDS tile chosen from it
-(on the device) span the same 5.4e9 cells. The photons start at (64.5, 64.5,
-64.5) and stay within 28 voxels of it for the simulated 0.5 ns, so the same photons in a 128^3
+in gate 4 lands at cell 2^32 or beyond; the pilot's count grid is as large, and the LDS tile is
+chosen from it on the device. The photons start at (64.5, 64.5, 64.5) and stay within 28 voxels
+of it for the simulated 0.5 ns, so the same photons in a 128^3
 cube (small enough for the oracle) follow the same trajectories: the final photon states must be
 bitwise equal, the big grid's [0,128)^3 window must hold the oracle's fluence and every other
 cell must stay zero.
