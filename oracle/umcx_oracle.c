@@ -8,7 +8,7 @@
  * FMA contraction on x86-64; see oracle/Makefile) and it calls the host libm where the
  * reference does, so on one host it follows the reference's trajectories bit for bit.
  *
- * Pinning (tests/test_oracle_*.py): the seed words against glibc rand() known answers;
+ * Pinning (tests/test_oracle.py): the seed words against glibc rand() known answers;
  * detected-photon rows, fluence and energies against golden vectors written by the real
  * reference binary (tests/golden/make_golden.py); closed forms (ballistic pencil beam,
  * normal-incidence Fresnel); energy balance.
