@@ -173,6 +173,12 @@ int umcx_config_from_argv(char** argv, int argn, char** json_out);
  * gate0, nx, ny, nz, ngates}, all zero for an empty grid. No counterpart in the reference. */
 int umcx_choose_tally_box(const float* counts, int32_t nx, int32_t ny, int32_t nz, int32_t ngates, uint64_t budget,
                           int32_t box[8]);
+/* The same search as run on the GPU by the sessions (densest cell, total and greedy growth on the
+ * device; the counts are copied there first); sums = {counts in the box, total} when not null.
+ * No counterpart in the reference: it lets a caller check the device search against
+ * umcx_choose_tally_box. */
+int umcx_choose_tally_box_device(const float* counts, int32_t nx, int32_t ny, int32_t nz, int32_t ngates,
+                                 uint64_t budget, int32_t box[8], double sums[2], int32_t device);
 /* glibc srand(seed)+4*rand() words used to seed every photon (src/umcx.cpp:725-727). */
 int umcx_seed_words(int32_t seed, uint32_t out[4]);
 /* Evaluate the libm replicas (fn 0 logf, 1 sinf, 2 cosf, 3 acosf, 4 expf) on the host CPU. */

@@ -80,6 +80,8 @@ SIGNATURES = {
     "umcx_config_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),
     "umcx_choose_tally_box": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                         C.POINTER(C.c_int32)]),
+    "umcx_choose_tally_box_device": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                               C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_int32]),
     "umcx_seed_words": (C.c_int, [C.c_int32, C.POINTER(C.c_uint32)]),
     "umcx_libm_host": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64]),
     "umcx_libm_device": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int32]),

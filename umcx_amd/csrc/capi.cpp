@@ -443,6 +443,18 @@ int umcx_choose_tally_box(const float* counts, int32_t nx, int32_t ny, int32_t n
     });
 }
 
+int umcx_choose_tally_box_device(const float* counts, int32_t nx, int32_t ny, int32_t nz, int32_t ngates,
+                                 uint64_t budget, int32_t box[8], double sums[2], int32_t device) {
+    return guarded([&]() -> int {
+        if (!counts || !box || nx <= 0 || ny <= 0 || nz <= 0 || ngates <= 0)
+            throw std::runtime_error("umcx_choose_tally_box_device: empty grid or null pointer");
+        const umcx::TallyBoxSearch r = umcx::device_choose_box(counts, nx, ny, nz, ngates, budget, device);
+        for (int i = 0; i < 8; i++) box[i] = r.box[i];
+        if (sums) { sums[0] = r.in_box; sums[1] = r.total; }
+        return 0;
+    });
+}
+
 int umcx_seed_words(int32_t seed, uint32_t out[4]) {
     if (!out) {
         g_last_error = "umcx_seed_words: null argument";

@@ -839,6 +839,22 @@ size_t Session::lds_bytes(uint32_t tile_n, bool paths_needed) const {
     return 32 * 16 + (size_t)prob_.mediumnum * 16 + (size_t)plabel_words_ * 4 + (size_t)tile_n * 4 + paths;
 }
 
+// The tally-box search on a count grid in device memory ([t][x][y][z], n cells): the densest
+// cell, the total, then the greedy growth in one workgroup. scratch: 8 device words (densest
+// cell, largest count bits, total, counts in the box, the box as 8 ints), cleared here.
+// Enqueued on s; read scratch back after the stream is synchronised.
+static void enqueue_box_search(const float* d_cnt, uint64_t n, int nt, int nx, int ny, int nz, uint64_t budget,
+                               unsigned long long* scratch, hipStream_t s) {
+    HIP_OK(hipMemsetAsync(scratch, 0, 64, s));
+    HIP_OK(hipMemsetAsync(scratch, 0xFF, 8, s));                       // densest cell starts at ~0
+    dev::count_max_kernel<<<2048, 256, 0, s>>>(d_cnt, n, (unsigned*)(scratch + 1));
+    dev::count_argmax_sum_kernel<<<2048, 256, 0, s>>>(d_cnt, n, (const unsigned*)(scratch + 1), scratch,
+                                                      (double*)(scratch + 2));
+    dev::grow_box_kernel<<<1, 1024, 0, s>>>(d_cnt, nt, nx, ny, nz, budget, scratch, (int*)(scratch + 4),
+                                            (double*)(scratch + 3));
+    HIP_OK(hipGetLastError());
+}
+
 // Pick the LDS tally tile from where a pilot sample of this photon range deposits most often.
 // The count grid never leaves the device: two passes find the densest cell (the first of equal
 // ones) and the total, and one workgroup grows the box from that cell exactly as choose_box
@@ -854,13 +870,11 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     const bool cache = n < 0xFFFFFFFFull;
     float* d_cnt = dalloc<float>(n);
     double* d_e = dalloc<double>(4);
-    // 0 cursor, 2-3 cache statistics, 4 densest cell, 5 largest count bits, 6 total, 7 in box,
-    // 8-11 the box (8 ints)
+    // 0 cursor, 2-3 cache statistics, 4-11 the box search
     unsigned long long* d_c = dalloc<unsigned long long>(12);
     HIP_OK(hipMemsetAsync(d_cnt, 0, n * 4, s));
     HIP_OK(hipMemsetAsync(d_e, 0, 32, s));
-    HIP_OK(hipMemsetAsync(d_c, 0, 96, s));
-    HIP_OK(hipMemsetAsync(d_c + 4, 0xFF, 8, s));                    // densest cell starts at ~0
+    HIP_OK(hipMemsetAsync(d_c, 0, 32, s));
     dev::KernelArgs ka;
     build_args(first, pilot, &ka);
     ka.fluence = d_cnt; ka.energy = d_e; ka.cursor = d_c; ka.trace = nullptr;
@@ -870,11 +884,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     const bool reflect = p.isreflect != 0;
     pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
-    dev::count_max_kernel<<<2048, 256, 0, s>>>(d_cnt, n, (unsigned*)(d_c + 5));
-    dev::count_argmax_sum_kernel<<<2048, 256, 0, s>>>(d_cnt, n, (const unsigned*)(d_c + 5), d_c + 4, (double*)(d_c + 6));
-    dev::grow_box_kernel<<<1, 1024, 0, s>>>(d_cnt, p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz, tile_budget_, d_c + 4,
-                                            (int*)(d_c + 8), (double*)(d_c + 7));
-    HIP_OK(hipGetLastError());
+    enqueue_box_search(d_cnt, n, p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz, tile_budget_, d_c + 4, s);
     unsigned long long head[12];
     HIP_OK(hipMemcpyAsync(head, d_c, sizeof head, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -1076,6 +1086,24 @@ uint64_t device_arith_check(int which, uint64_t* tested, int device) {
     (void)hipFree(bad);
     if (tested) *tested = n;
     return h;
+}
+
+TallyBoxSearch device_choose_box(const float* counts, int nx, int ny, int nz, int nt, uint64_t budget, int device) {
+    HIP_OK(hipSetDevice(device));
+    const uint64_t n = (uint64_t)nx * ny * nz * (uint64_t)nt;
+    float* d_cnt = dalloc<float>(n);
+    unsigned long long* d_s = dalloc<unsigned long long>(8);
+    HIP_OK(hipMemcpy(d_cnt, counts, n * 4, hipMemcpyHostToDevice));
+    enqueue_box_search(d_cnt, n, nt, nx, ny, nz, budget, d_s, nullptr);
+    unsigned long long h[8];
+    HIP_OK(hipMemcpy(h, d_s, sizeof h, hipMemcpyDeviceToHost));
+    (void)hipFree(d_cnt);
+    (void)hipFree(d_s);
+    TallyBoxSearch r;
+    memcpy(&r.total, &h[2], 8);
+    memcpy(&r.in_box, &h[3], 8);
+    memcpy(r.box, &h[4], sizeof r.box);
+    return r;
 }
 
 void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device) {

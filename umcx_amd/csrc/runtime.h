@@ -143,6 +143,14 @@ void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int devic
 // full ones (which: 0 sqrt, 1 reciprocal, 2 division); returns the number of differing inputs.
 uint64_t device_arith_check(int which, uint64_t* tested, int device);
 
+// The sessions' tally-box search (densest cell, total, greedy growth, all on the device) applied
+// to a host count grid [t][x][y][z].
+struct TallyBoxSearch {
+    int box[8];          // {x0, y0, z0, g0, sx, sy, sz, sg}, zeros for an empty grid
+    double in_box, total;
+};
+TallyBoxSearch device_choose_box(const float* counts, int nx, int ny, int nz, int nt, uint64_t budget, int device);
+
 int device_count();
 std::string device_name(int device);
 
