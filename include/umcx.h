@@ -147,7 +147,8 @@ int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n)
 int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]);
 /* One launch shape the autotuner timed at creation: workgroup size, persistent workgroups, LDS
  * tally cells per workgroup (0: global atomics only), the trial's kernel time over a 1/32 sample
- * of every job, the share of pilot deposits inside the jobs' tiles, and whether it was chosen. */
+ * of every job (0 when an explicit shape leaves a single candidate, which is not timed), the share
+ * of pilot deposits inside the jobs' tiles, and whether it was chosen. */
 typedef struct umcx_batch_trial {
     int32_t block, blocks, tile_cells, chosen;
     double ms, tile_share;

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -535,17 +536,18 @@ void BatchSession::tune() {
     // 1. counting pilot (no tile): deposits per photon and the deposit map of every job
     const int pblock = 256, pblocks = cus_ * std::max(1, occupancy(256, kLdsFixed));
     launch_once(pblock, std::min(pblocks, first_cap_), 0, nullptr, 1, kPilotPhotons, kPilotPhotons, 1);
-    std::vector<float> counts(flu_total_);
-    if (flu_total_) HIP_OK(hipMemcpy(counts.data(), d_fluence_, flu_total_ * 4, hipMemcpyDeviceToHost));
+    std::unique_ptr<float[]> counts(new float[std::max<uint64_t>(flu_total_, 1)]);   // filled by the copy
+    if (flu_total_) HIP_OK(hipMemcpy(counts.get(), d_fluence_, flu_total_ * 4, hipMemcpyDeviceToHost));
     std::vector<CountGrid> grids(probs_.size());
     for (size_t j = 0; j < probs_.size(); j++) {
         const Problem& p = probs_[j];
-        const uint64_t n = std::min<uint64_t>(p.nphoton, kPilotPhotons);
-        double tot = 0;
         const uint64_t len = fluence_len((int)j);
-        for (uint64_t i = 0; i < len; i++) tot += counts[flu_off_[j] + i];
-        cost_[j] = n ? tot / (double)n : 0.0;
-        grids[j] = CountGrid{len ? counts.data() + flu_off_[j] : nullptr, (int)p.nx, (int)p.ny, (int)p.nz, len ? p.maxgate : 0};
+        grids[j] = CountGrid{len ? counts.get() + flu_off_[j] : nullptr, (int)p.nx, (int)p.ny, (int)p.nz, len ? p.maxgate : 0};
+    }
+    const std::vector<double> totals = grid_totals(grids);
+    for (size_t j = 0; j < probs_.size(); j++) {
+        const uint64_t n = std::min<uint64_t>(probs_[j].nphoton, kPilotPhotons);
+        cost_[j] = n ? totals[j] / (double)n : 0.0;
     }
 
     // 2. candidates
@@ -585,9 +587,10 @@ void BatchSession::tune() {
     for (size_t j = 0; j < probs_.size(); j++) all_dep += cost_[j] * (double)probs_[j].nphoton;
     for (size_t k = 0; k < sizes.size(); k++) {
         box_sets[k] = choose_boxes(grids, (uint64_t)sizes[k]);
+        const std::vector<double> in_box = box_counts(grids, box_sets[k]);
         double in = 0;
         for (size_t j = 0; j < probs_.size(); j++)
-            if (grids[j].cnt) in += cost_[j] * (double)probs_[j].nphoton * box_share(grids[j], box_sets[k][j]);
+            if (totals[j] > 0) in += cost_[j] * (double)probs_[j].nphoton * (in_box[j] / totals[j]);
         shares[k] = all_dep > 0 ? in / all_dep : 0.0;
     }
     auto set_of = [&](int cells) -> const std::vector<TallyBox>* {
@@ -605,10 +608,13 @@ void BatchSession::tune() {
     // tiles are switched off: a tile also sums each workgroup's hot-cell deposits apart before
     // one add to HBM, which keeps the float32 tallies of the hottest voxels accurate (a voxel
     // under a pencil source takes ~1e7 single-precision adds otherwise).
+    // A single candidate (an explicit shape) needs no trial launch; its time is reported as 0.
     trials_.clear();
     for (const BatchTrial& c : cands) {
         BatchTrial t = c;
-        t.ms = launch_once(t.block, t.blocks, t.tile_cells, set_of(t.tile_cells), kTrialDiv, kTrialMin, ~0ull, 0);
+        t.ms = cands.size() > 1 ? launch_once(t.block, t.blocks, t.tile_cells, set_of(t.tile_cells), kTrialDiv,
+                                              kTrialMin, ~0ull, 0)
+                                : 0.0;
         t.tile_share = share_of(t.tile_cells);
         trials_.push_back(t);
     }

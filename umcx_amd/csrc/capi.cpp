@@ -330,9 +330,22 @@ void umcx_session_free(umcx_session* s) { delete s; }
 int umcx_batch_create(const char* const* jsons, int32_t njobs, const umcx_launch* launch, umcx_batch** out) {
     return guarded([&]() -> int {
         if (!jsons || njobs <= 0 || !out) throw std::runtime_error("umcx_batch_create: need at least one job");
-        std::vector<umcx::Problem> probs;
-        probs.reserve((size_t)njobs);
-        for (int32_t j = 0; j < njobs; j++) probs.push_back(umcx::make_problem(umcx::Json::parse(jsons[j])));
+        // jobs are parsed and their domains built in parallel; the first failing job's error is reported
+        std::vector<umcx::Problem> probs((size_t)njobs);
+        std::vector<std::string> errs((size_t)njobs);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int32_t j = 0; j < njobs; j++) {
+            try {
+                if (!jsons[j]) throw std::runtime_error("umcx_batch_create: null job config");
+                probs[j] = umcx::make_problem(umcx::Json::parse(jsons[j]));
+            } catch (const std::exception& e) {
+                errs[j] = e.what();
+            } catch (...) {
+                errs[j] = "unknown error";
+            }
+        }
+        for (int32_t j = 0; j < njobs; j++)
+            if (!errs[j].empty()) throw std::runtime_error("job " + std::to_string(j) + ": " + errs[j]);
         auto* b = new umcx_batch;
         try {
             b->b.reset(new umcx::BatchSession(probs, to_launch(launch)));

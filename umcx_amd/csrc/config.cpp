@@ -5,6 +5,7 @@
 
 #include <zlib.h>
 
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <ctime>
@@ -171,14 +172,14 @@ void glibc_seed_words(int32_t seed, uint32_t out[4]) {
 }
 
 std::vector<uint8_t> base64_decode(const std::string& s) {
-    static int8_t tab[256];
-    static bool init = false;
-    if (!init) {
-        memset(tab, -1, sizeof tab);
+    // built once, thread-safely (function-local static initialisation)
+    static const std::array<int8_t, 256> tab = [] {
+        std::array<int8_t, 256> t;
+        t.fill(-1);
         const char* a = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
-        for (int i = 0; i < 64; i++) tab[(uint8_t)a[i]] = (int8_t)i;
-        init = true;
-    }
+        for (int i = 0; i < 64; i++) t[(uint8_t)a[i]] = (int8_t)i;
+        return t;
+    }();
     std::vector<uint8_t> out;
     uint32_t acc = 0;
     int bits = 0;

@@ -91,6 +91,24 @@ std::vector<TallyBox> choose_boxes(const std::vector<CountGrid>& grids, uint64_t
     return out;
 }
 
+std::vector<double> grid_totals(const std::vector<CountGrid>& grids) {
+    std::vector<double> out(grids.size(), 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)grids.size(); i++) {
+        const CountGrid& g = grids[i];
+        if (g.cnt) out[i] = box_count(g, TallyBox{0, 0, 0, 0, g.nx, g.ny, g.nz, g.nt});
+    }
+    return out;
+}
+
+std::vector<double> box_counts(const std::vector<CountGrid>& grids, const std::vector<TallyBox>& boxes) {
+    std::vector<double> out(grids.size(), 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long i = 0; i < (long)grids.size(); i++)
+        if (grids[i].cnt) out[i] = box_count(grids[i], boxes[i]);
+    return out;
+}
+
 double box_count(const CountGrid& g, const TallyBox& b) {
     if (b[4] == 0) return 0.0;
     const uint64_t nyz = (uint64_t)g.ny * g.nz, nxyz = nyz * g.nx;

@@ -24,6 +24,10 @@ struct CountGrid {
 // choose_box for many grids at once (OpenMP over grids)
 std::vector<TallyBox> choose_boxes(const std::vector<CountGrid>& grids, uint64_t budget);
 
+// counts of each grid, and counts of each grid inside its box (OpenMP over grids; null grids: 0)
+std::vector<double> grid_totals(const std::vector<CountGrid>& grids);
+std::vector<double> box_counts(const std::vector<CountGrid>& grids, const std::vector<TallyBox>& boxes);
+
 // counts inside a box, and the fraction of a grid's counts that fall inside it
 double box_count(const CountGrid& g, const TallyBox& b);
 double box_share(const CountGrid& g, const TallyBox& b);
