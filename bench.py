@@ -241,7 +241,25 @@ def matched_noise_batch(jobs, chosen, sync, target: float = 0.01, reps: int = 8,
     }
 
 
-def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3) -> dict:
+def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3, target: float = 0.01) -> dict:
+    """The Jacobian comparison at 1e7 photons per source, then again at the photon count per
+    source that brings every pair's relative standard deviation of I_sd to the target (both
+    sides scale with the photon count, so the ratio is expected to hold)."""
+    import numpy as np
+
+    res = jacobian_at(sync, photons, fd_voxels)
+    worst = res["noise_rel_std_max"]
+    n1 = int(np.ceil(photons * 1.1 * (worst / target) ** 2 / 1e6)) * 1_000_000 if worst > target else photons
+    log(f"bench: jacobian at {n1:.3e} photons per source for <= {target:.0%} noise")
+    at = jacobian_at(sync, n1, 1)
+    res["at_1pct_noise"] = {k: at[k] for k in ("photons_per_source", "forward_ms", "replay_ms", "records",
+                                              "entries_per_s", "fd_plain_launch_ms", "speedup_vs_plain_loop",
+                                              "fd_vs_replay_max_rel_diff", "noise_rel_std_median",
+                                              "noise_rel_std_max")}
+    return res
+
+
+def jacobian_at(sync, photons: int, fd_voxels: int) -> dict:
     """d log I_sd / d(mua_v, mus_v) for 16 sources x 16 detectors on the 64x64x32 slab by one
     forward launch per source plus the replay kernel, against the loop of plain launches that
     central differences under common random numbers need (2 launches per (source, voxel,
@@ -302,8 +320,8 @@ def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3) -> dict:
     rep_ms = m.forward_ms + m.replay_ms
     return {
         "workload": "64x64x32 slab (mua 0.01, mus 1.0, g 0.9, n 1.37), 6x6x6 inclusion mua 0.035, 16 pencil "
-                    "sources x 16 disk detectors r=1.5 mm on 4x4 grids at z=0, 1e7 photons per source",
-        "entries": entries, "records": recs,
+                    f"sources x 16 disk detectors r=1.5 mm on 4x4 grids at z=0, {photons:.3g} photons per source",
+        "photons_per_source": photons, "entries": entries, "records": recs,
         "forward_ms": round(m.forward_ms, 2), "replay_ms": round(m.replay_ms, 2),
         "forward_wall_ms": round(fwd_wall, 2), "jacobian_wall_ms": round(jac_wall, 2),
         "entries_per_s": entries / (rep_ms * 1e-3),
