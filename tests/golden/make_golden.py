@@ -103,6 +103,15 @@ def zip_volume_case():
     return cfg
 
 
+# the longest domain axis the reference's short voxel coordinates allow (32767 voxels): a 3x3 bar
+# lit along x, nearly unscattering, so photons run its whole length, reflecting off its sides
+LONGX = {"Session": {"ID": "longx", "Photons": 2000, "RNGSeed": 4242, "DoMismatch": 1, "DoPartialPath": 0},
+         "Forward": {"T0": 0, "T1": 2e-7, "Dt": 2e-8},
+         "Domain": {"Dim": [32767, 3, 3], "LengthUnit": 1,
+                    "Media": [{"mua": 0, "mus": 0, "g": 1, "n": 1}, {"mua": 1e-5, "mus": 1e-4, "g": 0.9, "n": 1.37}]},
+         "Optode": {"Source": {"Type": "pencil", "Pos": [0.5, 1.5, 1.5], "Dir": [1, 0, 0]}},
+         "Shapes": [{"Grid": {"Tag": 1, "Size": [32767, 3, 3]}}]}
+
 CASES = {
     # name: (argv after the binary, json file contents or None)
     "c1": (["--bench", "cube60", "-d", "0", "-n", "20000", "-U", "0"], None),
@@ -129,6 +138,7 @@ CASES = {
                "-n", "20000", "-w", "53", "-U", "0", "-O", "e"], None),
     "normx": (["--bench", "cubesph60b", "-n", "5000", "-d", "0", "-u", "0.5"], None),
     "zerophotons": (["--bench", "cube60", "-n", "0"], None),
+    "longx": (["-f", "INPUT.json", "-U", "0"], LONGX),
 }
 FULL_FLUENCE_MAX = 300_000   # store the whole volume when it has at most this many cells
 SAMPLE = 4096

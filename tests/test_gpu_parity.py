@@ -23,11 +23,11 @@ from umcx_amd import bjdata
 pytestmark = pytest.mark.gpu
 
 TRACE_CASES = ["c1", "cube60b", "c2", "c3", "isotropic", "cone", "cube60planar", "fourier", "sphshells",
-               "spherebox", "zipvol", "c4src", "skinvessel", "gated"]
+               "spherebox", "zipvol", "c4src", "skinvessel", "gated", "longx"]
 DETP_CASES = ["cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "c2", "c4src", "zipvol",
               "fourier", "save2pt0", "gated"]
 FLUENCE_CASES = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel",
-                 "c2", "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "gated", "normx"]
+                 "c2", "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "gated", "normx", "longx"]
 
 
 def say(msg):

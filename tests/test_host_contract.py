@@ -27,7 +27,7 @@ from umcx_amd import _native, bjdata
 ROOT = O.ROOT
 GOLDEN = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel", "c2",
           "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx",
-          "zerophotons"]
+          "zerophotons", "longx"]
 BENCH_ARGV = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel",
               "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated", "normx", "zerophotons"]
 

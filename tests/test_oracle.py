@@ -21,7 +21,7 @@ import oracle_lib as O
 
 GOLDEN_CASES = ["c1", "cube60", "cube60b", "cubesph60b", "cube60planar", "sphshells", "spherebox", "skinvessel",
                 "c2", "c3", "c4src", "zipvol", "isotropic", "cone", "fourier", "save2pt0", "overflow", "gated",
-                "normx"]
+                "normx", "longx"]
 
 
 def sorted_rows(rows):
