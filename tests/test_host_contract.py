@@ -185,6 +185,26 @@ def test_batch_without_gpu_fails_loudly():
         B.BatchSimulation(B.uneven_jobs(seed=7, njobs=2, max_photons=100))
 
 
+def test_batch_job_errors_name_the_job():
+    """Jobs are parsed and their domains built in parallel; a bad job's error names it (the first
+    bad one when several are), before any GPU work."""
+    import ctypes as C
+    import json as _json
+    from umcx_amd import batch as B
+    from umcx_amd._native import UmcxLaunch, lib
+    jobs = B.uneven_jobs(seed=7, njobs=48, max_photons=100)
+    jobs[17]["Domain"]["Dim"] = [40000, 10, 10]
+    del jobs[31]["Session"]["Photons"]
+    texts = [_json.dumps(j).encode() for j in jobs]
+    arr = (C.c_char_p * len(texts))(*texts)
+    h = C.c_void_p()
+    assert lib().umcx_batch_create(arr, len(texts), C.byref(UmcxLaunch()), C.byref(h)) == 1
+    assert U.last_error() == "job 17: Domain dimensions must be within 1..32767 voxels per axis"
+    arr[17] = None
+    assert lib().umcx_batch_create(arr, len(texts), C.byref(UmcxLaunch()), C.byref(h)) == 1
+    assert U.last_error() == "job 17: umcx_batch_create: null job config"
+
+
 def test_batch_rejects_an_empty_job_list():
     import ctypes as C
     from umcx_amd._native import UmcxLaunch, lib
