@@ -10,6 +10,10 @@
  * Session part: the photon loop MCX_kernel (src/umcx.cpp:722-781) as a reusable GPU session,
  * so a caller can prepare a problem once, run photon ranges on the GPU, and read tallies,
  * detected photons and energies back without going through files.
+ *
+ * Threads: the drop-in exports and separate sessions or batches may be used from several threads
+ * at once (each session and batch owns its device buffers and stream; error messages are
+ * per thread). One session or batch handle must not be used by two threads at the same time.
  */
 #ifndef UMCX_H
 #define UMCX_H

@@ -295,6 +295,35 @@ def test_photon_ranges_compose():
     assert split_e[0] == pytest.approx(full_e[0], rel=1e-9)
 
 
+def test_sessions_run_concurrently_from_threads():
+    """Separate sessions used from several host threads at once (the C calls release the GIL):
+    every session's detected rows, energies and fluence equal a run of the same session alone."""
+    from concurrent.futures import ThreadPoolExecutor
+    names = ["cube60b", "c2", "spherebox", "c4src"]
+
+    def one(name):
+        g = O.golden(name)
+        with U.Simulation(g["cfg"]) as sim:
+            out = []
+            for _ in range(3):
+                sim.reset()
+                sim.run()
+                out.append((sorted_rows(sim.detp()[0]), sim.energies(), sim.fluence()))
+            return out
+
+    say("sessions alone")
+    alone = {n: one(n) for n in names}
+    say("sessions from 4 threads at once")
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        together = dict(zip(names, ex.map(one, names)))
+    for n in names:
+        rows0, e0, f0 = alone[n][0]
+        for rows, e, f in alone[n][1:] + together[n]:
+            assert np.array_equal(rows.view(np.uint32), rows0.view(np.uint32)), n
+            assert e[1] == e0[1] and e[0] == pytest.approx(e0[0], rel=1e-10), n
+            assert np.linalg.norm((f - f0).ravel()) <= 1e-5 * np.linalg.norm(f0.ravel()), n
+
+
 def test_launch_shape_does_not_change_results():
     g = O.golden("cube60b")
     outs = []
