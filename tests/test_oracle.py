@@ -33,7 +33,7 @@ def test_seed_words_match_glibc_rand():
         kat = json.load(f)
     for seed, words in kat.items():
         assert O.seed_words(int(seed)) == words
-    # the survey's published values for the default seed
+    # glibc srand(1648335518) followed by four rand() calls (the default RNGSeed's words)
     assert O.seed_words(1648335518) == [1017983987, 166281306, 1334304517, 1099791551]
 
 
