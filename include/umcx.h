@@ -63,7 +63,9 @@ typedef struct umcx_info {
     float tstart, tend, rtstep, unitinmm;
     uint32_t seeds[4];
     int32_t block, blocks, tally_mode, label_bytes;
-    int32_t lds_label_bits;  /* bits per voxel of the LDS-staged labels (0: labels read from HBM) */
+    int32_t lds_label_bits;  /* bits per voxel of the LDS-staged labels (0: labels read from HBM);
+                                a volume of at most two media ids may switch from 2 or 4 to 1
+                                after the pilot, for a larger tally tile */
     int32_t last_block, last_blocks;  /* launch shape of the last run (0 before any): with no explicit
                                          shape, runs under 12 photons per lane of the full shape use
                                          half the lanes */

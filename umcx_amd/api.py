@@ -99,6 +99,13 @@ class Simulation:
         check(lib().umcx_session_info(self._h, C.byref(info)))
         return info.last_block, info.last_blocks
 
+    def lds_label_bits(self) -> int:
+        """Bits per voxel of the labels staged in LDS now (0: read from HBM); the pilot may switch a
+        volume of at most two media ids to its 1-bit form."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return int(info.lds_label_bits)
+
     def deposit_cache(self) -> bool:
         """Whether runs use the per-lane cache of global fluence deposits."""
         info = UmcxInfo()

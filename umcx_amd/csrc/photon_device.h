@@ -84,17 +84,20 @@ struct GlobalLabels {
         return v;
     }
 };
-// LdsLabels: media ids packed 2 or 4 bits per voxel into 32-bit words staged in LDS (no detector
-// flag: the kernels read that one from HBM when a photon ends).
+// LdsLabels: media ids packed 1, 2 or 4 bits per voxel into 32-bit words staged in LDS (no
+// detector flag: the kernels read that one from HBM when a photon ends). The media id is
+// id0 + code * step: 2- and 4-bit codes are the id itself (id0 0, step 1); 1-bit codes, used when
+// the volume holds at most two media ids, select the smaller (code 0) or the larger one.
 struct LdsLabels {
     const uint32_t* w;
-    uint32_t lg;      // log2(voxels per word): 4 for 2-bit, 3 for 4-bit labels
-    uint32_t bits;    // 2 or 4
+    uint32_t lg;      // log2(voxels per word): 5, 4 or 3 for 1-, 2- or 4-bit labels
+    uint32_t bits;    // 1, 2 or 4
+    uint32_t id0, step;
     __device__ __forceinline__ uint32_t raw(int idx) const {
         const uint32_t i = (uint32_t)idx;
         return (w[i >> lg] >> ((i & ((1u << lg) - 1u)) * bits)) & ((1u << bits) - 1u);
     }
-    __device__ __forceinline__ uint32_t decode(uint32_t v) const { return v; }
+    __device__ __forceinline__ uint32_t decode(uint32_t v) const { return id0 + v * step; }
 };
 
 template <typename LabelT>

@@ -39,7 +39,7 @@
 namespace umcx {
 
 constexpr uint64_t kLabelCacheBytes = 96 * 1024;   // largest packed label volume staged in LDS
-constexpr uint64_t kMaxTileCells = 32768;          // largest LDS tally tile
+constexpr uint64_t kMaxTileCells = 40960;          // largest LDS tally tile
 constexpr int kRegPaths = 4;                       // partial paths kept in registers up to this many media
 // below this many photons per lane of the full launch shape a run ends with its slowest photons,
 // and it runs on half the lanes so that fewer waves share each SIMD
@@ -89,9 +89,10 @@ struct KernelArgs {
     uint64_t gstride;
     float unit;              // mm per voxel
     float* rw;               // final weight of each replayed photon (may be null)
-    // media ids packed plabel_bits (2 or 4) per voxel, staged in LDS (LDSLAB kernels)
+    // media ids packed plabel_bits (1, 2 or 4) per voxel, staged in LDS (LDSLAB kernels); code c
+    // stands for medium plabel_id0 + c * plabel_step
     const uint32_t* plabels;
-    uint32_t plabel_words, plabel_lg, plabel_bits;
+    uint32_t plabel_words, plabel_lg, plabel_bits, plabel_id0, plabel_step;
     // photon slots: workgroup w first takes [w*per_wg, (w+1)*per_wg) through an LDS counter,
     // then slots from blocks*per_wg on through the device cursor
     uint64_t per_wg;
@@ -111,7 +112,7 @@ template <typename LabelT>
 struct LabelPick<true, LabelT> {
     using T = LdsLabels;
     static __device__ __forceinline__ T make(const KernelArgs& a, const uint32_t* s) {
-        return T{s, a.plabel_lg, a.plabel_bits};
+        return T{s, a.plabel_lg, a.plabel_bits, a.plabel_id0, a.plabel_step};
     }
 };
 
@@ -708,18 +709,57 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
         }
         d_labels_ = dalloc<uint8_t>(nvox);
         HIP_OK(hipMemcpy(d_labels_, lab.data(), nvox, hipMemcpyHostToDevice));
-        // media ids packed 2 or 4 bits per voxel for LDS staging, when the whole volume fits
-        const uint32_t bits = p.mediumnum <= 4 ? 2u : (p.mediumnum <= 16 ? 4u : 0u);
-        const uint64_t words = bits ? (nvox * bits + 31) / 32 : 0;
-        if (lc.label_cache >= 0 && bits && words * 4 <= kLabelCacheBytes) {
+        // media ids packed for LDS staging, when the whole volume fits: 2 or 4 bits holding the id
+        // (up to 4 or 16 media); a volume holding at most two media ids also gets a 1-bit form
+        // (code 0 = the smaller id), which leaves more LDS to the tally tile and is switched to
+        // after the pilot when most deposits would otherwise go to HBM (or used from the start
+        // when it is the only form that fits)
+        uint32_t ids[2] = {~0u, ~0u};
+        int nids = 0;
+        for (uint64_t i = 0; i < nvox && nids <= 2; i++) {
+            const uint32_t m = (uint32_t)p.labels[i] & kMedMask;
+            if (nids > 0 && m == ids[0]) continue;
+            if (nids > 1 && m == ids[1]) continue;
+            if (nids < 2) ids[nids] = m;
+            nids++;
+        }
+        if (nids == 2 && ids[1] < ids[0]) std::swap(ids[0], ids[1]);
+        if (nids == 1) ids[1] = ids[0];
+        auto pack = [&](uint32_t bits, uint64_t* words_out) -> uint32_t* {
+            const uint64_t words = (nvox * bits + 31) / 32;
+            if (lc.label_cache < 0 || words * 4 > kLabelCacheBytes) return nullptr;
             std::vector<uint32_t> packed(words, 0u);
             const uint32_t per = 32 / bits;
-            for (uint64_t i = 0; i < nvox; i++)
-                packed[i / per] |= ((uint32_t)p.labels[i] & kMedMask) << ((i % per) * bits);
-            d_plabels_ = dalloc<uint32_t>(words);
-            HIP_OK(hipMemcpy(d_plabels_, packed.data(), words * 4, hipMemcpyHostToDevice));
+            for (uint64_t i = 0; i < nvox; i++) {
+                uint32_t code = (uint32_t)p.labels[i] & kMedMask;
+                if (bits == 1) code = code == ids[0] ? 0u : 1u;
+                packed[i / per] |= code << ((i % per) * bits);
+            }
+            uint32_t* d = dalloc<uint32_t>(words);
+            HIP_OK(hipMemcpy(d, packed.data(), words * 4, hipMemcpyHostToDevice));
+            *words_out = words;
+            return d;
+        };
+        const uint32_t wbits = p.mediumnum <= 4 ? 2u : (p.mediumnum <= 16 ? 4u : 0u);
+        uint64_t words = 0;
+        if (wbits && (d_plabels_ = pack(wbits, &words))) {
             plabel_words_ = (uint32_t)words;
-            plabel_bits_ = bits;
+            plabel_bits_ = wbits;
+        }
+        if (nids <= 2) {
+            uint32_t* d1 = pack(1, &words);
+            if (d1 && !d_plabels_) {                 // the 1-bit form is the only one that fits
+                d_plabels_ = d1;
+                plabel_words_ = (uint32_t)words;
+                plabel_bits_ = 1;
+                plabel_id0_ = ids[0];
+                plabel_step_ = ids[1] - ids[0];
+            } else if (d1) {
+                d_plabels1_ = d1;
+                plabel1_words_ = (uint32_t)words;
+                plabel1_id0_ = ids[0];
+                plabel1_step_ = ids[1] - ids[0];
+            }
         }
     } else {
         d_labels_ = dalloc<uint32_t>(nvox);
@@ -760,8 +800,8 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     // LDS tally tile: share the CU's LDS between the workgroups the register budget lets in
     tally_mode_ = lc.tally_mode;
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 160 * 1024;
-    size_t per_wg = std::min(lds_cu / (size_t)per_cu, lds_wg_max);
-    tile_budget_ = per_wg > lds ? std::min<uint64_t>((per_wg - lds) / 4, kMaxTileCells) : 0;
+    per_wg_lds_ = std::min(lds_cu / (size_t)per_cu, lds_wg_max);
+    tile_budget_ = per_wg_lds_ > lds ? std::min<uint64_t>((per_wg_lds_ - lds) / 4, kMaxTileCells) : 0;
     if (tally_mode_ == 1 && tile_budget_ < 64) tile_budget_ = 64;
     blocks_ = lc.blocks > 0 ? lc.blocks : prop.multiProcessorCount * per_cu;
     if (p.threadnum > 0 && lc.blocks <= 0) blocks_ = std::max(1, p.threadnum / block_);
@@ -774,7 +814,7 @@ Session::~Session() {
     (void)hipSetDevice(lc_.device);
     for (void* ptr : {(void*)d_labels_, (void*)d_media_, (void*)d_detpos_, (void*)d_fluence_, (void*)d_detp_,
                       (void*)d_detid_, (void*)d_detw_, (void*)d_counters_, (void*)d_cursor_, (void*)d_energy_,
-                      (void*)d_trace_, d_vprops_, (void*)d_plabels_, (void*)d_msrc_})
+                      (void*)d_trace_, d_vprops_, (void*)d_plabels_, (void*)d_plabels1_, (void*)d_msrc_})
         if (ptr) (void)hipFree(ptr);
     if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
     if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
@@ -821,7 +861,9 @@ void* Session::build_args(uint64_t first, uint64_t count, void* out) const {
     ka.plabels = d_plabels_;
     ka.plabel_words = plabel_words_;
     ka.plabel_bits = plabel_bits_;
-    ka.plabel_lg = plabel_bits_ == 2 ? 4u : 3u;
+    ka.plabel_lg = plabel_bits_ == 1 ? 5u : (plabel_bits_ == 2 ? 4u : 3u);
+    ka.plabel_id0 = plabel_id0_;
+    ka.plabel_step = plabel_step_;
     ka.per_wg = (count - count / 8) / (uint64_t)std::max(blocks_, 1);
     ka.msrc_n = msrc_n_;
     ka.rsrcs = msrc_n_ ? d_msrc_ : nullptr;
@@ -888,12 +930,32 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     unsigned long long head[12];
     HIP_OK(hipMemcpyAsync(head, d_c, sizeof head, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
-    (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
     dep_hits_ = head[2] ? 1.0 - (double)head[3] / (double)head[2] : 0.0;
     double total, in_box;
     memcpy(&total, &head[6], 8);
     memcpy(&in_box, &head[7], 8);
     memcpy(tile_, &head[8], sizeof(int) * 8);
+    // most deposits outside the tile: where the volume has a 1-bit label form, take it and the
+    // larger tile the LDS it frees allows (no change to any photon, only to where it tallies)
+    if (d_plabels1_ && (total > 0 ? in_box / total : 0.0) < kCacheBelowShare) {
+        const size_t lds1 = lds_bytes(0) - (size_t)plabel_words_ * 4 + (size_t)plabel1_words_ * 4;
+        const uint64_t budget1 =
+            per_wg_lds_ > lds1 ? std::min<uint64_t>((per_wg_lds_ - lds1) / 4, kMaxTileCells) : 0;
+        if (budget1 > tile_budget_) {
+            enqueue_box_search(d_cnt, n, p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz, budget1, d_c + 4, s);
+            HIP_OK(hipMemcpyAsync(head, d_c, sizeof head, hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            memcpy(&in_box, &head[7], 8);
+            memcpy(tile_, &head[8], sizeof(int) * 8);
+            std::swap(d_plabels_, d_plabels1_);
+            std::swap(plabel_words_, plabel1_words_);
+            plabel_bits_ = 1;
+            plabel_id0_ = plabel1_id0_;
+            plabel_step_ = plabel1_step_;
+            tile_budget_ = budget1;
+        }
+    }
+    (void)hipFree(d_cnt); (void)hipFree(d_e); (void)hipFree(d_c);
     // the per-lane deposit cache pays where most deposits go to HBM and a fair share of them hit
     // it (the pilot ran with it); otherwise its extra work per deposit costs more than it saves
     dep_cache_ = (total > 0 ? in_box / total : 0.0) < kCacheBelowShare && dep_hits_ >= kCacheMinHits;

@@ -124,7 +124,11 @@ private:
     float* d_msrc_ = nullptr;            // multi-source table {pos xyz, dir xyz} per source
     uint64_t msrc_n_ = 0;                // photons per source (0: one source)
     uint32_t* d_plabels_ = nullptr;      // packed media ids for LDS staging (null: not staged)
-    uint32_t plabel_words_ = 0, plabel_bits_ = 0;
+    uint32_t plabel_words_ = 0, plabel_bits_ = 0, plabel_id0_ = 0, plabel_step_ = 1;
+    // the 1-bit form of the staged labels, kept beside the 2-/4-bit one until the pilot picks
+    uint32_t* d_plabels1_ = nullptr;
+    uint32_t plabel1_words_ = 0, plabel1_id0_ = 0, plabel1_step_ = 0;
+    size_t per_wg_lds_ = 0;     // LDS per workgroup at the session's launch shape
 
     void* build_args(uint64_t first, uint64_t count, void* out) const;
     size_t lds_bytes(uint32_t tile_n, bool paths_needed = true) const;   // dynamic LDS of a launch
