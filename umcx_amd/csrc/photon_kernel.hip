@@ -103,6 +103,20 @@ struct KernelArgs {
     uint64_t msrc_n;
 };
 
+// The kernel's arguments read again from the kernarg segment where they are used, through a
+// pointer the compiler cannot see is invariant: scalar loads at the point of use instead of
+// scalar registers held (and spilled to VGPR lanes) across the whole photon loop.
+typedef const __attribute__((address_space(4))) KernelArgs* ArgsRef;
+__device__ __forceinline__ ArgsRef reread_args() {
+    uint64_t v = (uint64_t)__builtin_amdgcn_kernarg_segment_ptr();
+    __asm__ volatile("" : "+s"(v));
+    return (ArgsRef)v;
+}
+__device__ __forceinline__ float4 ld4(const __attribute__((address_space(4))) float4* q) {
+    const __attribute__((address_space(4))) float* f = (const __attribute__((address_space(4))) float*)q;
+    return make_float4(f[0], f[1], f[2], f[3]);
+}
+
 template <bool LDSLAB, typename LabelT>
 struct LabelPick {
     using T = GlobalLabels<LabelT>;
@@ -153,7 +167,6 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     if (tid == 0) s_taken = 0u;
     const Tabs tabs{slog, sexp};
     const Grid grid{a.nx, a.ny, a.nz, a.nyz};
-    Source src{a.srcpos, a.srcdir, a.srcp1, a.srcp2, a.srctype};
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
     if (LDSLAB)
         for (uint32_t i = tid; i < a.plabel_words; i += bd) slab[i] = a.plabels[i];
@@ -195,6 +208,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         // The workgroup's static slots come through an LDS counter, so a refill does not wait on
         // a device atomic (whose wait would also cover every fluence atomic still in flight).
         if (!alive) {
+            const ArgsRef ca = reread_args();
+            Source src{ld4(&ca->srcpos), ld4(&ca->srcdir), ld4(&ca->srcp1), ld4(&ca->srcp2), ca->srctype};
             const unsigned long long need = __ballot(1);
             const int leader = __ffsll((long long)need) - 1;
             const uint32_t n = (uint32_t)__popcll(need), rank = (uint32_t)__popcll(need & lanemask_lt());
@@ -202,16 +217,16 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             unsigned long long gb = 0;
             if ((int)__lane_id() == leader) {
                 sb = atomicAdd(&s_taken, n);
-                nstat = sb < a.per_wg ? (uint32_t)min((uint64_t)n, a.per_wg - sb) : 0u;
-                if (nstat < n) gb = atomicAdd(a.cursor, (unsigned long long)(n - nstat));
+                nstat = sb < ca->per_wg ? (uint32_t)min((uint64_t)n, ca->per_wg - sb) : 0u;
+                if (nstat < n) gb = atomicAdd(ca->cursor, (unsigned long long)(n - nstat));
             }
             sb = __shfl(sb, leader);
             nstat = __shfl(nstat, leader);
             gb = __shfl(gb, leader);
-            const unsigned long long k = rank < nstat ? (uint64_t)blockIdx.x * a.per_wg + sb + rank
-                                                      : (uint64_t)gridDim.x * a.per_wg + gb + (rank - nstat);
-            if (k >= a.count) break;
-            pid = REPLAY ? (uint64_t)a.rids[k] : a.first + k;
+            const unsigned long long k = rank < nstat ? (uint64_t)blockIdx.x * ca->per_wg + sb + rank
+                                                      : (uint64_t)gridDim.x * ca->per_wg + gb + (rank - nstat);
+            if (k >= ca->count) break;
+            pid = REPLAY ? (uint64_t)ca->rids[k] : ca->first + k;
             if (REPLAY) {
                 rk = k;
                 rcoef = a.rcoef[k];
@@ -224,15 +239,15 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 }
             }
             uint64_t ip = pid;                       // photon index within its source
-            if (!REPLAY && a.msrc_n) {
-                const uint64_t si = pid / a.msrc_n;
-                ip = pid - si * a.msrc_n;
-                const float* q = a.rsrcs + 6 * si;
+            if (!REPLAY && ca->msrc_n) {
+                const uint64_t si = pid / ca->msrc_n;
+                ip = pid - si * ca->msrc_n;
+                const float* q = ca->rsrcs + 6 * si;
                 src.srcpos.x = q[0]; src.srcpos.y = q[1]; src.srcpos.z = q[2];
                 src.srcdir.x = q[3]; src.srcdir.y = q[4]; src.srcdir.z = q[5];
             }
             const uint32_t i32 = (uint32_t)ip;
-            rng_seed(r, a.seeds[0] ^ i32, a.seeds[1] | i32, a.seeds[2] ^ i32, a.seeds[3] | i32);   // :761
+            rng_seed(r, ca->seeds[0] ^ i32, ca->seeds[1] | i32, ca->seeds[2] ^ i32, ca->seeds[3] | i32);   // :761
             w0 = launch(p, r, src);
             ph_dep = 0.0;
             if (SAVEDET) {
@@ -246,8 +261,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             if (!entered) {
                 p.med = 0;
                 alive = false;
-                if (a.trace) {
-                    TraceRec& t = a.trace[k];
+                if (ca->trace) {
+                    TraceRec& t = ca->trace[k];
                     t.pos[0] = p.px; t.pos[1] = p.py; t.pos[2] = p.pz; t.pos[3] = p.pw;
                     t.vec[0] = p.vx; t.vec[1] = p.vy; t.vec[2] = p.vz; t.vec[3] = p.vw;
                     t.len[0] = p.lx; t.len[1] = p.ly; t.len[2] = p.lz; t.len[3] = p.lw;
@@ -354,25 +369,26 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         }
         // ---- photon ends: detection (:276-278) and energy tallies (:765-770) ----
         if (SAVEDET && p.ly <= a.tend && det_flag(p)) {
-            for (int d = 0; d < a.detnum; d++) {
-                const float4 dp = a.detpos[d];
+            const ArgsRef cd = reread_args();
+            for (int d = 0; d < cd->detnum; d++) {
+                const float4 dp = cd->detpos[d];
                 float dx = dp.x - p.px, dy = dp.y - p.py, dz = dp.z - p.pz;
                 if (dx * dx + dy * dy + dz * dz < dp.w * dp.w) {
-                    uint32_t slot = atomicAdd(a.ndet, 1u);
-                    if (slot < a.maxdet) {
-                        float* row = a.detp + (uint64_t)slot * a.detlen;
+                    uint32_t slot = atomicAdd(cd->ndet, 1u);
+                    if (slot < cd->maxdet) {
+                        float* row = cd->detp + (uint64_t)slot * cd->detlen;
                         int c = 0;
-                        if (a.savedetflag & kDetID) row[c++] = (float)(d + 1);
+                        if (cd->savedetflag & kDetID) row[c++] = (float)(d + 1);
                         if (regpp) {
                             const float pp[kRegPaths] = {pp0, pp1, pp2, pp3};
                             for (int m = 0; m < a.ppathlen; m++) row[c++] = pp[m];
                         } else {
                             for (int m = 0; m < a.ppathlen; m++) row[c++] = sppath[m * bd + tid];
                         }
-                        if (a.savedetflag & kExitPos) { row[c++] = p.px; row[c++] = p.py; row[c++] = p.pz; }
-                        if (a.savedetflag & kExitDir) { row[c++] = p.vx; row[c++] = p.vy; row[c++] = p.vz; }
-                        a.detid[slot] = (uint32_t)pid;
-                        a.detw[slot] = p.pw;
+                        if (cd->savedetflag & kExitPos) { row[c++] = p.px; row[c++] = p.py; row[c++] = p.pz; }
+                        if (cd->savedetflag & kExitDir) { row[c++] = p.vx; row[c++] = p.vy; row[c++] = p.vz; }
+                        cd->detid[slot] = (uint32_t)pid;
+                        cd->detw[slot] = p.pw;
                     }
                 }
             }
@@ -382,8 +398,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             e_esc += (double)p.pw;
             e_dep += ph_dep;
         }
-        if (a.trace) {
-            TraceRec& t = a.trace[pid - a.first];
+        if (reread_args()->trace) {
+            TraceRec& t = reread_args()->trace[pid - reread_args()->first];
             t.pos[0] = p.px; t.pos[1] = p.py; t.pos[2] = p.pz; t.pos[3] = p.pw;
             t.vec[0] = p.vx; t.vec[1] = p.vy; t.vec[2] = p.vz; t.vec[3] = p.vw;
             t.len[0] = p.lx; t.len[1] = p.ly; t.len[2] = p.lz; t.len[3] = p.lw;
