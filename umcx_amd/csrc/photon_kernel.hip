@@ -44,6 +44,7 @@ constexpr int kRegPaths = 4;                       // partial paths kept in regi
 // below this many photons per lane of the full launch shape a run ends with its slowest photons,
 // and it runs on half the lanes so that fewer waves share each SIMD
 constexpr uint64_t kShortRunPhotonsPerLane = 12;
+constexpr int kHbmLabelWgPerCu = 4;                // workgroups per CU when labels are read from HBM
 constexpr int kDepCache = 2;                       // global deposits cached per lane
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
@@ -813,6 +814,9 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
     HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0), block_, lds));
     if (per_cu < 1) per_cu = 1;
+    // labels in HBM: at most kHbmLabelWgPerCu workgroups per CU, so each keeps a 40 KiB LDS share
+    // for its tally tile (200^3 skin model: 2256 ms at 4 per CU, 2304 ms at the 5 registers allow)
+    if (!plabel_words_) per_cu = std::min(per_cu, kHbmLabelWgPerCu);
     // LDS tally tile: share the CU's LDS between the workgroups the register budget lets in
     tally_mode_ = lc.tally_mode;
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 160 * 1024;
