@@ -41,19 +41,19 @@ FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 (vector/matrix) peak
 # VALU issue: a wave64 FP32 instruction occupies a 16-lane SIMD for 4 cycles, so 256 CUs x 4 SIMDs
 # x 2.4 GHz / 4 = 6.14e11 wave-instructions/s (an FMA-chain probe reaches 6.03e11). The photon
 # kernel's instruction count per photon and active-lane fraction on this workload come from
-# rocprofv3 counters (profiles/photon_kernel_hetero_reflective_1e8.md: SQ_INSTS_VALU 4.411e10 and
-# SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU) = 0.398 over 1e7 photons).
+# rocprofv3 counters (profiles/photon_kernel_hetero_reflective_1e8.md: SQ_INSTS_VALU 4.701e10 and
+# SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU) = 0.402 over 1e7 photons).
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
-VALU_INSTR_PER_PHOTON = 4411.0
-ACTIVE_LANE_FRAC = 0.398
+VALU_INSTR_PER_PHOTON = 4701.0
+ACTIVE_LANE_FRAC = 0.402
 # HBM traffic per photon of the same kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over 1e7
-# photons: 1,107 KiB and 35,401,593 KiB, scaled by 2.0 and 1.0 -- what the two counters read for
+# photons: 1,106 KiB and 33,799,634 KiB, scaled by 2.0 and 1.0 -- what the two counters read for
 # a 256 MiB device-to-device copy); almost all of it is the memory-side float atomics' 32-byte
 # writes.
-HBM_BYTES_PER_PHOTON = (1107 * 2.0 + 35_401_593 * 1.0) * 1024 / 1e7
-# memory-side float atomics per photon (TCC_ATOMIC 1.132e9 over 1e7 photons) against the rate
+HBM_BYTES_PER_PHOTON = (1106 * 2.0 + 33_799_634 * 1.0) * 1024 / 1e7
+# memory-side float atomics per photon (TCC_ATOMIC 1.081e9 over 1e7 photons) against the rate
 # umcx_amd/bin/roofline_probe measures for hashed addresses (profiles/roofline_probe_gfx950.md)
-ATOMICS_PER_PHOTON = 113.2
+ATOMICS_PER_PHOTON = 108.1
 ATOMIC_PEAK = 2.11e10
 
 CONFIG = {

@@ -51,7 +51,7 @@ typedef struct umcx_launch {
     int32_t label_cache; /* 0: media ids packed into LDS when the volume fits (<= 96 KiB at 2 or 4 bits
                             per voxel; 1 bit after the pilot for at most two media ids, see
                             umcx_info.lds_label_bits), -1: labels always read from HBM */
-    int32_t deposit_cache; /* 0: per-lane cache of the last 2 global fluence deposits when the LDS tile
+    int32_t deposit_cache; /* 0: per-lane cache of the last 4 global fluence deposits when the LDS tile
                               takes under half of them (pilot count), 1: always, -1: never */
 } umcx_launch;
 

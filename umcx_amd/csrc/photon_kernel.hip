@@ -14,7 +14,7 @@
 // fluence deposits are also bitwise equal to the reference's, up to atomic summation order.
 //
 // Fluence deposits go to an LDS tally box (chosen from a counting pilot), else through a
-// per-lane cache of the last two global deposits, else straight to HBM as float atomics, which
+// per-lane cache of the last four global deposits, else straight to HBM as float atomics, which
 // execute at the memory side and are the kernel's binding ceiling.
 #include <hip/hip_runtime.h>
 
@@ -45,7 +45,7 @@ constexpr int kRegPaths = 4;                       // partial paths kept in regi
 // and it runs on half the lanes so that fewer waves share each SIMD
 constexpr uint64_t kShortRunPhotonsPerLane = 12;
 constexpr int kHbmLabelWgPerCu = 4;                // workgroups per CU when labels are read from HBM
-constexpr int kDepCache = 2;                       // global deposits cached per lane
+constexpr int kDepCache = 4;                       // global deposits cached per lane
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
 
