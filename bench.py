@@ -11,10 +11,10 @@ resident inputs (the label volume and media are uploaded once, outside the timed
 Prints one JSON line. ``roofline`` prices the photon kernel's algorithmic FP32 work (per-photon
 op model of the photon loop, counted from src/umcx.cpp:315-428: 34 per step, 9 per voxel crossing,
 62 per scatter, 1 per random draw, 35 per Fresnel test) against the MI355X FP32 peak, using the
-kernel's HIP-event time on its own stream. Beside it, ``roofline.atomics`` gives the ceiling
-that binds -- memory-side float atomics per second against the rate umcx_amd/bin/roofline_probe
-measures -- and ``roofline.valu_issue`` the VALU instruction issue rate (instructions per photon
-from rocprofv3 counters) with the active-lane fraction. ``cpu_baseline`` times the reference umcx binary built
+kernel's HIP-event time on its own stream. Beside it are the two ceilings the kernel runs
+against: ``roofline.atomics`` -- memory-side float atomics per second against the rate
+umcx_amd/bin/roofline_probe measures -- and ``roofline.valu_issue``, the VALU instruction issue
+rate (instructions per photon from rocprofv3 counters) with the active-lane fraction. ``cpu_baseline`` times the reference umcx binary built
 from source (or, when it is absent, the CPU oracle) on this machine's host cores over a bounded
 sample of the same workload.
 """
