@@ -39,8 +39,11 @@ int umcx_device_count(void);
 
 typedef struct umcx_session umcx_session;
 
-/* Launch options; zero fields take tuned defaults. No counterpart in the reference, whose
- * ThreadNum/BlockSize session keys (src/umcx.cpp:739-741) are honoured when these are zero. */
+/* Launch options. A NULL pointer takes the tuned defaults for every field. In a caller-filled
+ * struct, block, blocks, label_cache and deposit_cache take their tuned defaults at 0, but
+ * tally_mode and vote must be -1 for theirs (0 selects global atomics only and immediate
+ * scattering). No counterpart in the reference, whose ThreadNum/BlockSize session keys
+ * (src/umcx.cpp:739-741) are honoured when block and blocks are zero. */
 typedef struct umcx_launch {
     int32_t device;      /* HIP device ordinal */
     int32_t block;       /* threads per workgroup */
@@ -122,7 +125,9 @@ int umcx_session_replay(umcx_session* s, const uint32_t* ids, const int32_t* sli
 
 /* Several sources in one run: sources = {pos xyz, dir xyz} per source (host), photon slot p of
  * umcx_session_run is photon p mod photons_per_source of source p / photons_per_source, seeded
- * and traced exactly as in that source's own run; detection records carry the slot. NULL or
+ * and traced exactly as in that source's own run; detection records carry the slot. While a
+ * table is set, umcx_session_run rejects (before any launch) a range [first, first+count) that
+ * ends past nsrc * photons_per_source; nsrc * photons_per_source is at most 2^32. NULL or
  * nsrc 0 returns to the single source. No counterpart in the reference. */
 int umcx_session_set_sources(umcx_session* s, const float* sources, int32_t nsrc, uint64_t photons_per_source);
 /* umcx_session_replay for the records of several sources in one launch: record k is launched

@@ -243,3 +243,25 @@ def test_several_sources_in_one_run_equal_separate_runs():
             got = np.column_stack([rows[m], (slots[m] - s * N).astype(np.float32), w[m]])
             ref = np.column_stack([r1, i1.astype(np.float32), w1])
             assert np.array_equal(srt(got).view(np.uint32), srt(ref).view(np.uint32)), s
+
+
+def test_run_past_the_last_source_is_rejected_before_launch():
+    """With a source table set, a photon range that ends past nsrc x photons_per_source is an
+    error returned by umcx_session_run on the host; no kernel is launched for it."""
+    from umcx_amd._native import check, lib
+    g = O.golden("c4src")
+    cfg = copy.deepcopy(g["cfg"])
+    srcs = J.grid_sources(64, 64, 2, 16.5)
+    N = 1000
+    say("source table range check")
+    table = np.array([list(p) + list(d) for p, d in srcs], dtype=np.float32).reshape(-1)
+    with U.Simulation(cfg, tally_mode=0) as sim:
+        check(lib().umcx_session_set_sources(sim._h, table.ctypes.data, len(srcs), N))
+        total = len(srcs) * N
+        for first, count in ((0, total + 1), (total, 1), (total - 5, 6), (2 ** 63, 2 ** 63)):
+            rc = lib().umcx_session_run(sim._h, first, count, None)
+            assert rc != 0, (first, count)
+            assert b"exceed" in lib().umcx_last_error(), (first, count)
+        # nothing ran: tallies are still zero
+        assert sim.energies()[1] == 0.0
+        check(lib().umcx_session_set_sources(sim._h, None, 0, 0))

@@ -1000,6 +1000,15 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     const Problem& p = prob_;
     RunStats st;
     st.photons = count;
+    // with several sources, slot p reads source p / msrc_n_ of the table: a range past the last
+    // source would read beyond it
+    if (msrc_n_) {
+        const uint64_t total = msrc_count_ * msrc_n_;
+        if (first > total || count > total - first)
+            throw std::runtime_error("umcx_session_run: photons [" + std::to_string(first) + ", " +
+                                     std::to_string(first + count) + ") exceed the " + std::to_string(msrc_count_) +
+                                     " sources x " + std::to_string(msrc_n_) + " photons set by umcx_session_set_sources");
+    }
     last_count_ = count;
     if (lc_.trace) {
         if (trace_cap_ < count) {
@@ -1067,12 +1076,14 @@ void Session::set_sources(const float* table, int nsrc, uint64_t per_source) {
     if (d_msrc_) HIP_OK(hipFree(d_msrc_));
     d_msrc_ = nullptr;
     msrc_n_ = 0;
+    msrc_count_ = 0;
     if (!table || nsrc <= 0 || per_source == 0) return;
     if ((uint64_t)nsrc * per_source > 0xFFFFFFFFull)
         throw std::runtime_error("several sources in one run: at most 2^32 photons in total");
     d_msrc_ = dalloc<float>((size_t)nsrc * 6);
     HIP_OK(hipMemcpy(d_msrc_, table, (size_t)nsrc * 6 * 4, hipMemcpyHostToDevice));
     msrc_n_ = per_source;
+    msrc_count_ = (uint64_t)nsrc;
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
     dep_cache_ = true;

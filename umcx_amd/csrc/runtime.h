@@ -123,6 +123,7 @@ private:
     uint64_t tile_budget_ = 0;
     float* d_msrc_ = nullptr;            // multi-source table {pos xyz, dir xyz} per source
     uint64_t msrc_n_ = 0;                // photons per source (0: one source)
+    uint64_t msrc_count_ = 0;            // sources in the table
     uint32_t* d_plabels_ = nullptr;      // packed media ids for LDS staging (null: not staged)
     uint32_t plabel_words_ = 0, plabel_bits_ = 0, plabel_id0_ = 0, plabel_step_ = 1;
     // the 1-bit form of the staged labels, kept beside the 2-/4-bit one until the pilot picks
