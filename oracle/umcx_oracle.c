@@ -341,15 +341,59 @@ static int boundary(or_ctx* c, or_photon* ph, float n1, float n2, or_rng* g, int
     return 0;
 }
 
-/* MCX_photon::sprint (:280-314): returns 1 when the photon terminates */
-static int fly(or_ctx* c, or_photon* ph, or_rng* g, float* ppath, double* edep) {
+/* Per-voxel segment accounting of the replay (oracle_replay): at every voxel change the path
+ * since the previous change (len.z, in voxels) and the scatterings taken meanwhile are credited
+ * to the voxel the photon leaves. */
+typedef struct {
+    double coef;                  /* record coefficient */
+    double* gmua; double* gmus;   /* gradient slice of the record */
+    double* amua; double* amus;   /* sums of |term| per entry (may be NULL) */
+    float* mpath;                 /* per-medium sums of the segments, float, in crossing order */
+    const float* vprops;          /* nvox x {mua, mus} per voxel length, or NULL: the labels' media */
+    float unit;                   /* mm per voxel */
+    uint32_t nscat;               /* scatterings since the last voxel change */
+} or_seg;
+
+static inline void seg_add(double* a, double v) {
+#pragma omp atomic
+    *a += v;
+}
+
+static void seg_flush(or_seg* sg, const or_ctx* c, const or_photon* ph, int32_t nv) {
+    const int32_t v = (g_mutant == 4 && nv >= 0) ? nv : ph->lastvox;   /* mutant 4: credit the new voxel */
+    const float mus_vox = sg->vprops ? sg->vprops[2 * (size_t)v + 1] : medium(c, ph->med)[1];
+    const double L = (double)ph->len.z * (double)sg->unit;              /* mm */
+    const double tmua = -sg->coef * L;
+    const double tmus = sg->coef * ((sg->nscat ? (double)sg->nscat * (double)sg->unit / (double)mus_vox : 0.0) - L);
+    seg_add(sg->gmua + v, tmua);
+    seg_add(sg->gmus + v, tmus);
+    if (sg->amua) seg_add(sg->amua + v, fabs(tmua));
+    if (sg->amus) seg_add(sg->amus + v, fabs(sg->coef) * ((sg->nscat ? (double)sg->nscat * (double)sg->unit / (double)mus_vox : 0.0) + L));
+    const uint32_t m = (uint32_t)ph->med & OR_MED_MASK;
+    if (sg->mpath && m > 0) sg->mpath[m - 1] += ph->len.z;
+    sg->nscat = 0;
+}
+
+/* MCX_photon::sprint (:280-314): returns 1 when the photon terminates. With `sg` (replay), every
+ * voxel change is also credited to the segment accounts and len.z restarts there. */
+static int fly_seg(or_ctx* c, or_photon* ph, or_rng* g, float* ppath, double* edep, or_seg* sg) {
     const oracle_problem* p = c->p;
     while (ph->len.x > 0.f) {
         const float* prop = medium(c, ph->med);
+        float vp[4];
+        if (sg && sg->vprops) {                    /* per-voxel coefficients replace the label's */
+            vp[0] = sg->vprops[2 * (size_t)ph->lastvox]; vp[1] = sg->vprops[2 * (size_t)ph->lastvox + 1];
+            vp[2] = prop[2]; vp[3] = prop[3];
+            prop = vp;
+        }
         int32_t nv = step(c, ph, prop);
         if (nv == ph->lastvox) continue;
         ph->ncross++;
         uint32_t m = (uint32_t)ph->med & OR_MED_MASK;
+        if (sg) {
+            seg_flush(sg, c, ph, nv);
+            ph->len.z = 0.f;
+        }
         if (p->issavedet && (p->savedetflag & DP_PPATH) && m > 0) ppath[m - 1] += ph->len.z;
         if (p->issavevol) {
             int tshift = (int)fminf((float)(p->maxgate - 1), (float)(int)floorf((ph->len.y - p->tstart) * p->rtstep));
@@ -367,6 +411,10 @@ static int fly(or_ctx* c, or_photon* ph, or_rng* g, float* ppath, double* edep) 
         ph->med = nm;
     }
     return 0;
+}
+
+static int fly(or_ctx* c, or_photon* ph, or_rng* g, float* ppath, double* edep) {
+    return fly_seg(c, ph, g, ppath, edep, NULL);
 }
 
 /* MCX_detect::addphoton (:164-176) called from savedetector (:429-435) */
@@ -465,5 +513,90 @@ int oracle_run(const oracle_problem* p, uint64_t first, uint64_t count, float* f
     energies[0] += esc;
     energies[1] += tot;
     energies[2] += dep;
+    return 0;
+}
+
+/* Replay for the differentiable form (perturbation Monte Carlo per voxel). Re-traces photon
+ * ids[k] -- photon i is a pure function of i (:761) -- launched from source src_index[k] of
+ * `sources` ({pos xyz, dir xyz} per source; NULL: the problem's source), and credits every voxel
+ * segment (the path between two voxel changes, as sprint :285-292 delimits it) to the voxel the
+ * photon leaves:
+ *   gmua[slices[k]*stride + v] += coef[k] * (-L)
+ *   gmus[slices[k]*stride + v] += coef[k] * (n / mus_v - L)
+ * with L the segment length in mm, n the scatterings taken in it (:364-380) and mus_v in 1/mm.
+ * Summed over a photon's segments these are d w / d mua_v / w and d log p / d mus_v of its final
+ * weight w and path density p. amua (may be NULL) receives the sums of the mua terms' absolute
+ * values per entry, amus (may be NULL) the sums of |coef| (n / mus_v + L), the magnitudes the
+ * mus terms are formed from (the scale of their rounding); final_w (may be NULL) the replayed final weights; mpath (may be NULL,
+ * mediumnum floats per record) the per-medium sums of the segment lengths in voxels, accumulated
+ * in float in crossing order like the reference's partial paths (:286-288). vprops (may be NULL)
+ * holds per-voxel {mua, mus} in per-voxel-length units replacing the labels' (g and n stay per
+ * label). Accumulation in double. Returns 0, or -1 on invalid input. */
+int oracle_replay(const oracle_problem* p0, const uint32_t* ids, const int32_t* src_index, const float* sources,
+                  const int32_t* slices, const double* coef, uint64_t n, const float* vprops, double* gmua,
+                  double* gmus, double* amua, double* amus, uint64_t stride, float* final_w, float* mpath,
+                  int nthreads) {
+    if (!p0 || !p0->labels || !p0->media || p0->mediumnum <= 0 || !ids || !slices || !coef || !gmua || !gmus) return -1;
+    oracle_problem pr = *p0;
+    pr.issavevol = 0;           /* the replay tallies no fluence and saves no detector rows */
+    pr.issavedet = 0;
+    or_ctx c;
+    memset(&c, 0, sizeof c);
+    c.p = &pr;
+    c.dimyz = (uint64_t)pr.ny * pr.nz;
+    c.dimxyz = c.dimyz * pr.nx;
+    uint32_t dummy = 0;
+    c.ndet = &dummy;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const int nmed = pr.mediumnum;
+    int64_t nn = (int64_t)n;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t k = 0; k < nn; k++) {
+        oracle_problem ps = pr;
+        if (sources && src_index) {
+            const float* q = sources + 6 * (size_t)src_index[k];
+            ps.srcpos[0] = q[0]; ps.srcpos[1] = q[1]; ps.srcpos[2] = q[2];
+            ps.srcdir[0] = q[3]; ps.srcdir[1] = q[4]; ps.srcdir[2] = q[5];
+        }
+        or_ctx ck = c;
+        ck.p = &ps;
+        const uint64_t i = ids[k];
+        or_seg sg;
+        memset(&sg, 0, sizeof sg);
+        sg.coef = coef[k];
+        sg.gmua = gmua + (uint64_t)slices[k] * stride;
+        sg.gmus = gmus + (uint64_t)slices[k] * stride;
+        sg.amua = amua ? amua + (uint64_t)slices[k] * stride : NULL;
+        sg.amus = amus ? amus + (uint64_t)slices[k] * stride : NULL;
+        sg.mpath = mpath ? mpath + (size_t)k * nmed : NULL;
+        if (sg.mpath) memset(sg.mpath, 0, sizeof(float) * (size_t)nmed);
+        sg.vprops = vprops;
+        sg.unit = ps.unitinmm;
+        or_rng g;
+        or_photon ph;
+        memset(&ph, 0, sizeof ph);
+        g.ndraw = 0;
+        rng_seed(&g, ps.seeds[0] ^ (uint32_t)i, ps.seeds[1] | (uint32_t)i, ps.seeds[2] ^ (uint32_t)i, ps.seeds[3] | (uint32_t)i);
+        (void)launch(&ph, &g, &ps);
+        double edep = 0.0;
+        float pp[1] = {0.f};
+        ph.lastvox = vox_index(&ck, ph.ix, ph.iy, ph.iz);
+        int entered = 1;
+        if (ph.lastvox < 0 && skip_to_box(&ck, &ph) < 0.f) entered = 0;
+        if (entered && ph.lastvox < 0) entered = 0;
+        if (entered) {
+            ph.med = ps.labels[ph.lastvox];
+            ph.len.x = rng_scatlen(&g);
+            while (fly_seg(&ck, &ph, &g, pp, &edep, &sg) == 0) {
+                scatter(&ph, &g, medium(&ck, ph.med));
+                sg.nscat++;
+            }
+        }
+        if (final_w) final_w[k] = ph.pos.w;
+    }
     return 0;
 }

@@ -72,7 +72,21 @@ int oracle_run(const oracle_problem* p, uint64_t first, uint64_t count, float* f
                float* detp, uint32_t* detp_id, float* detp_w, uint32_t* ndetected,
                double energies[3], oracle_trace* traces, int nthreads);
 
-/* Select a planted mistake (0 = none); used only by tests that prove the checks have teeth. */
+/* Replay of detected photons for the differentiable form: per-voxel perturbation-MC terms of
+ * record k, in voxel v of the path, added into slice slices[k] (stride floats apart):
+ *   gmua += coef[k] * (-L_v)     gmus += coef[k] * (n_v / mus_v - L_v)
+ * L_v: path length in voxel v (mm), n_v: scatterings in v, mus_v in 1/mm. Photon ids[k] is
+ * launched from source src_index[k] of sources ({pos xyz, dir xyz}; NULL: the problem's).
+ * amua (sums of |mua term|), amus (sums of |coef| (n_v / mus_v + L_v)), final_w, mpath
+ * (per-medium float segment sums, mediumnum per record) and vprops
+ * (per-voxel {mua, mus} per voxel length) may be NULL. Returns 0, or -1 on invalid input. */
+int oracle_replay(const oracle_problem* p, const uint32_t* ids, const int32_t* src_index, const float* sources,
+                  const int32_t* slices, const double* coef, uint64_t n, const float* vprops, double* gmua,
+                  double* gmus, double* amua, double* amus, uint64_t stride, float* final_w, float* mpath,
+                  int nthreads);
+
+/* Select a planted mistake (0 = none); used only by tests that prove the checks have teeth.
+ * 4 credits each replay segment to the voxel the photon enters instead of the one it leaves. */
 void oracle_set_mutant(int m);
 
 /* Length of a detected-photon row for this problem (:157). */

@@ -63,6 +63,10 @@ def lib():
         _lib.oracle_detp_len.argtypes = [C.POINTER(OracleProblem)]
         _lib.oracle_host_libm_array.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64]
         _lib.oracle_set_mutant.argtypes = [C.c_int]
+        _lib.oracle_replay.argtypes = [C.POINTER(OracleProblem), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_int]
+        _lib.oracle_replay.restype = C.c_int
         for fn in ("logf", "sinf", "cosf", "acosf", "expf"):
             f = getattr(_lib, "oracle_host_" + fn)
             f.argtypes = [C.c_float]
@@ -153,6 +157,60 @@ def problem_from_config(cfg: dict, labels: np.ndarray) -> dict:
                           dtype=np.float32).reshape(-1, 4),
         labels=np.ascontiguousarray(labels, dtype=np.int32),
     )
+
+
+def _oracle_problem(prob: dict) -> OracleProblem:
+    nx, ny, nz = prob["dims"]
+    op = OracleProblem()
+    op.nx, op.ny, op.nz = nx, ny, nz
+    op.maxgate = prob["maxgate"]
+    op.tstart, op.tend, op.rtstep, op.unitinmm = prob["tstart"], prob["tend"], prob["rtstep"], prob["unitinmm"]
+    op.isreflect, op.issavevol, op.issavedet, op.savedetflag = (prob["isreflect"], prob["issavevol"],
+                                                                 prob["issavedet"], prob["savedetflag"])
+    media, detpos, labels = prob["media"], prob["detpos"], prob["labels"]
+    op.mediumnum, op.outputtype, op.detnum, op.srctype = len(media), prob["outputtype"], len(detpos), prob["srctype"]
+    op.maxdetphotons = prob["maxdetphotons"]
+    for name in ("srcpos", "srcdir", "srcparam1", "srcparam2"):
+        arr = getattr(op, name)
+        for i, v in enumerate(prob[name][:4]):
+            arr[i] = float(v)
+    for i, v in enumerate(prob["seeds"]):
+        op.seeds[i] = v
+    op.labels = labels.ctypes.data_as(C.POINTER(C.c_int32))
+    op.media = media.ctypes.data_as(C.POINTER(C.c_float))
+    op.detpos = detpos.ctypes.data_as(C.POINTER(C.c_float)) if len(detpos) else C.POINTER(C.c_float)()
+    return op
+
+
+def replay(prob: dict, ids, slices, coef, nslices: int, sources=None, src_index=None, vprops=None,
+           nthreads: int = 0) -> dict:
+    """Per-voxel perturbation-MC replay of photons ``ids`` (oracle_replay): returns gmua, gmus
+    (nslices x nvox float64: sum of coef x (-L_v) and coef x (n_v/mus_v - L_v)), amua (sums of the
+    mua terms' absolute values), amus (sums of |coef| (n_v/mus_v + L_v), the scale of the mus
+    terms' rounding), final_w and mpath (per-medium float32 segment sums in voxels,
+    one row per record). ``sources`` is a (S, 6) {pos, dir} table indexed by ``src_index``;
+    ``vprops`` per-voxel (nvox, 2) {mua, mus} in per-voxel-length units."""
+    op = _oracle_problem(prob)
+    nvox = int(np.prod(prob["dims"]))
+    ids = np.ascontiguousarray(ids, dtype=np.uint32)
+    n = len(ids)
+    slices = np.ascontiguousarray(slices, dtype=np.int32)
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    gmua = np.zeros((nslices, nvox)); gmus = np.zeros((nslices, nvox))
+    amua = np.zeros((nslices, nvox)); amus = np.zeros((nslices, nvox))
+    final_w = np.zeros(max(n, 1), dtype=np.float32)
+    mpath = np.zeros((max(n, 1), len(prob["media"])), dtype=np.float32)
+    srcs = np.ascontiguousarray(sources, dtype=np.float32).reshape(-1) if sources is not None else None
+    sidx = np.ascontiguousarray(src_index, dtype=np.int32) if src_index is not None else None
+    vp = np.ascontiguousarray(vprops, dtype=np.float32).reshape(-1) if vprops is not None else None
+    rc = lib().oracle_replay(C.byref(op), ids.ctypes.data, sidx.ctypes.data if sidx is not None else None,
+                             srcs.ctypes.data if srcs is not None else None, slices.ctypes.data, coef.ctypes.data, n,
+                             vp.ctypes.data if vp is not None else None, gmua.ctypes.data, gmus.ctypes.data,
+                             amua.ctypes.data, amus.ctypes.data, nvox, final_w.ctypes.data, mpath.ctypes.data,
+                             nthreads)
+    if rc != 0:
+        raise RuntimeError("oracle_replay rejected its input")
+    return dict(gmua=gmua, gmus=gmus, amua=amua, amus=amus, final_w=final_w[:n], mpath=mpath[:n])
 
 
 def run(prob: dict, nphoton: int | None = None, first: int = 0, traces: bool = False, nthreads: int = 0):

@@ -191,8 +191,14 @@ def test_autograd_function_matches_vjp():
     Jf = m.jacobian()                                   # (S*D, nvox, 2)
     ref_mua = (wts.reshape(-1, 1) * Jf[:, :, 0]).sum(0).reshape(mua.shape)
     ref_mus = (wts.reshape(-1, 1) * Jf[:, :, 1]).sum(0).reshape(mus.shape)
-    torch.testing.assert_close(mua.grad, ref_mua, rtol=1e-4, atol=1e-6 * float(ref_mua.abs().max()))
-    torch.testing.assert_close(mus.grad, ref_mus, rtol=1e-4, atol=1e-6 * float(ref_mus.abs().max()))
+    # both sides are float32 atomic sums in different orders; the mus terms (n/mus - L) have both
+    # signs, so an entry's rounding error scales with the weighted sum of the slices' magnitudes,
+    # not with the entry itself
+    for k, (got, ref) in enumerate(((mua.grad, ref_mua), (mus.grad, ref_mus))):
+        scale = (wts.abs().reshape(-1, 1) * Jf[:, :, k].abs()).sum(0).reshape(ref.shape)
+        bound = 2e-4 * scale + 1e-6 * float(ref.abs().max())
+        excess = float(((got - ref).abs() - bound).max())
+        assert excess <= 0.0, (k, excess)
     m.close()
 
 
@@ -265,3 +271,41 @@ def test_run_past_the_last_source_is_rejected_before_launch():
         # nothing ran: tallies are still zero
         assert sim.energies()[1] == 0.0
         check(lib().umcx_session_set_sources(sim._h, None, 0, 0))
+
+
+@pytest.mark.parametrize("nside,photons", [(1, 200000), (2, 100000)])
+def test_jacobian_equals_oracle_replay_entry_by_entry(nside, photons):
+    """Every entry of the GPU Jacobian (1 or 4 sources x 16 detectors x 131072 voxels x {mua,
+    mus}) against the oracle's per-voxel replay of the same detected photons with the same float32
+    coefficients (w / I_sd), accumulated in float64. The oracle also sums the magnitudes each
+    entry is formed from (|coef| L for mua, |coef| (n/mus + L) for mus); float32 rounding of the
+    terms and of the atomic sums stays below 2e-5 of that scale, and entries no segment reaches
+    are exactly zero on both sides."""
+    g = O.golden("c4src")
+    srcs = J.grid_sources(64, 64, nside, 16.5)
+    say(f"jacobian vs oracle replay, {len(srcs)} sources x {photons} photons")
+    m = J.MultiSourceModel(g["cfg"], srcs, photons)
+    m.forward()
+    Jg = m.jacobian().cpu().numpy()                     # (S*D, nvox, 2)
+    S, D = len(srcs), m.ndet
+    ids, sidx, sl, cf = [], [], [], []
+    for s, (i, det, w) in enumerate(m.records):
+        Isd = m.I[s, det]
+        coef = np.where(Isd > 0, w / np.where(Isd > 0, Isd, 1.0), 0.0).astype(np.float32)
+        ids.append(i.astype(np.uint32)); sidx.append(np.full(len(i), s, np.int32))
+        sl.append((s * D + det).astype(np.int32)); cf.append(coef.astype(np.float64))
+    m.close()
+    lab = U.domain(m.cfg)
+    prob = O.problem_from_config(m.cfg, lab)
+    table = np.array([list(p) + list(d) for p, d in srcs], dtype=np.float32)
+    r = O.replay(prob, np.concatenate(ids), np.concatenate(sl), np.concatenate(cf), S * D, sources=table,
+                 src_index=np.concatenate(sidx), nthreads=8)
+    nrec = sum(len(x) for x in ids)
+    assert nrec > 1000
+    for k, (ora, scale) in enumerate(((r["gmua"], r["amua"]), (r["gmus"], r["amus"]))):
+        gpu = Jg[:, :, k].astype(np.float64)
+        untouched = scale == 0
+        assert np.all(gpu[untouched] == 0.0), k
+        ratio = np.abs(gpu[~untouched] - ora[~untouched]) / scale[~untouched]
+        say(f"  param {k}: {int((~untouched).sum())} entries, max |gpu - oracle| / scale = {ratio.max():.3e}")
+        assert ratio.max() <= 2e-5, (k, ratio.max())

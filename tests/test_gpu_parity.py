@@ -406,6 +406,32 @@ def test_deposit_cache_changes_only_summation_order():
     assert out[1][1][1] == out[-1][1][1]
     a, b = out[1][2], out[-1][2]
     assert np.linalg.norm(a - b) <= 1e-4 * np.linalg.norm(b)
+    # nothing lost in eviction or the final flush: each deposit is the weight drop dw divided by
+    # the voxel's mua (fluence output, every medium here has mua >= FLT_EPSILON), so the cached
+    # tallies times mua sum to the kernel's fp64 deposited weight
+    lab = U.domain(g["cfg"]) & 0x7FFFFFFF
+    mua = O.problem_from_config(g["cfg"], U.domain(g["cfg"]))["media"][:, 0].astype(np.float64)
+    per_vox = a.sum(axis=0).reshape(-1)
+    assert float((per_vox * mua[lab.reshape(-1)]).sum()) == pytest.approx(out[1][1][2], rel=1e-5)
+
+
+def test_deposit_cache_fluence_matches_reference_golden():
+    """Forced on, the deposit cache's fluence equals the reference binary's at the fluence bar."""
+    g, prob = O.golden_problem("c2")
+    say("deposit cache on vs reference golden c2")
+    with U.Simulation(g["cfg"], deposit_cache=1) as sim:
+        assert sim.deposit_cache()
+        sim.run()
+        res = sim.result(normalize=bool(prob["isnormalized"]))
+    f = res.fluence.reshape(-1)
+    ref = g["fluence"] if "fluence" in g else None
+    if ref is not None:
+        assert np.linalg.norm(f - ref) / np.linalg.norm(ref) <= 1e-4
+    else:
+        idx, val = g["sample_idx"], g["sample_val"]
+        assert np.linalg.norm(f[idx] - val) / np.linalg.norm(val) <= 1e-4
+    np.testing.assert_allclose(res.fluence.astype(np.float64).sum(axis=(1, 2, 3)), g["gate_sums"], rtol=1e-4,
+                               atol=1e-30)
 
 
 def test_full_size_heterogeneous_reflective_properties():
