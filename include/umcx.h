@@ -172,6 +172,45 @@ int umcx_batch_trials(const umcx_batch* b, umcx_batch_trial* out, int32_t cap, i
 int umcx_batch_tally_box(const umcx_batch* b, int32_t job, int32_t box[8], double* deposits_per_photon);
 void umcx_batch_free(umcx_batch* b);
 
+/* ---- one photon range on caller-owned device buffers, with no allocation ------------------- */
+
+/* A problem held entirely in caller-owned device memory. The constants are those of
+ * umcx_session_info for the same config (media in per-voxel-length units, as MCX_param and the
+ * media scaling of src/umcx.cpp:785-810 derive them); every pointer is a device pointer. */
+typedef struct umcx_device_problem {
+    uint32_t dims[4];          /* nx, ny, nz, maxgate */
+    int32_t mediumnum;         /* media rows, label 0 included: 1..16 */
+    int32_t detnum;            /* detector rows */
+    int32_t isreflect, issavedet, savedetflag, outputtype, srctype;
+    float tstart, tend, rtstep, unitinmm;
+    float srcpos[4], srcdir[4], srcparam1[4], srcparam2[4];
+    uint32_t seeds[4];         /* seed words (umcx_seed_words of RNGSeed) */
+    uint32_t maxdetphotons;    /* rows the detp / detid / detw buffers hold */
+    int32_t deposit_cache;     /* 1: per-lane cache of the last 4 global fluence deposits, 0: none */
+    const uint8_t* labels;     /* nx*ny*nz, [x][y][z]: bits 0-6 medium id (< mediumnum), bit 7 detector mask */
+    const float* media;        /* mediumnum x {mua, mus, g, n} */
+    const float* detpos;       /* detnum x {x, y, z, r}, grid units */
+    float* fluence;            /* maxgate*nx*ny*nz raw tallies [t][x][y][z], added to; NULL: no volume output */
+    float* detp;               /* maxdetphotons x detp_columns rows (src/umcx.cpp:164-176) */
+    uint32_t* detid;           /* photon index of each row */
+    float* detw;               /* final weight of each row */
+    uint32_t* ndet;            /* detections, incl. ones past maxdetphotons; added to */
+    double* energies;          /* {escaped, launched, deposited}, added to */
+} umcx_device_problem;
+
+/* Photons [first, first+count) of umcx_session_run's kernel on the buffers of *p, enqueued on
+ * `stream` (a hipStream_t of the current device; NULL: the null stream) without waiting and
+ * without allocating anything: no hipMalloc, no hipFree, no memset node, so it may be captured in
+ * a HIP graph. The caller zeroes the outputs before the first range. No counterpart in the
+ * reference: it exists for callers that cannot allocate (graph capture, serving loops, harnesses
+ * that forbid it). The work cursor and the LDS-staged media ids live in one static device buffer
+ * of 98,560 bytes per GPU, shared by every call on that GPU: calls that overlap on one GPU race,
+ * and volumes whose media ids packed at 2 bits (<= 4 media) or 4 bits (<= 16 media) per voxel
+ * exceed 96 KiB are rejected (use a session). No LDS tally tile is used; results equal
+ * umcx_session_run's up to float summation order. Errors are reported before anything is
+ * enqueued. */
+int umcx_run_device_noalloc(const umcx_device_problem* p, uint64_t first, uint64_t count, void* stream);
+
 /* ---- host-only helpers (no GPU needed) ---------------------------------------------------- */
 
 /* Label volume with detector bits, as MCX_userio::initdomain + maskdetectors build it

@@ -35,6 +35,20 @@ class UmcxBatchTrial(C.Structure):
                 ("ms", C.c_double), ("tile_share", C.c_double)]
 
 
+class UmcxDeviceProblem(C.Structure):
+    """umcx_device_problem of include/umcx.h (every pointer a device pointer)."""
+    _fields_ = [("dims", C.c_uint32 * 4), ("mediumnum", C.c_int32), ("detnum", C.c_int32),
+                ("isreflect", C.c_int32), ("issavedet", C.c_int32), ("savedetflag", C.c_int32),
+                ("outputtype", C.c_int32), ("srctype", C.c_int32),
+                ("tstart", C.c_float), ("tend", C.c_float), ("rtstep", C.c_float), ("unitinmm", C.c_float),
+                ("srcpos", C.c_float * 4), ("srcdir", C.c_float * 4), ("srcparam1", C.c_float * 4),
+                ("srcparam2", C.c_float * 4), ("seeds", C.c_uint32 * 4), ("maxdetphotons", C.c_uint32),
+                ("deposit_cache", C.c_int32),
+                ("labels", C.c_void_p), ("media", C.c_void_p), ("detpos", C.c_void_p), ("fluence", C.c_void_p),
+                ("detp", C.c_void_p), ("detid", C.c_void_p), ("detw", C.c_void_p), ("ndet", C.c_void_p),
+                ("energies", C.c_void_p)]
+
+
 # every exported symbol and its signature: (restype, argtypes)
 SIGNATURES = {
     "MCX_run_cmd": (C.c_int, [C.POINTER(C.c_char_p), C.c_int]),
@@ -76,6 +90,7 @@ SIGNATURES = {
     "umcx_batch_trials": (C.c_int, [C.c_void_p, C.POINTER(UmcxBatchTrial), C.c_int32, C.POINTER(C.c_int32)]),
     "umcx_batch_tally_box": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "umcx_batch_free": (None, [C.c_void_p]),
+    "umcx_run_device_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p]),
     "umcx_domain_from_json": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_void_p)]),
     "umcx_config_from_argv": (C.c_int, [C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_void_p)]),
     "umcx_choose_tally_box": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_uint64,

@@ -421,6 +421,10 @@ int umcx_batch_tally_box(const umcx_batch* b, int32_t job, int32_t box[8], doubl
 
 void umcx_batch_free(umcx_batch* b) { delete b; }
 
+int umcx_run_device_noalloc(const umcx_device_problem* p, uint64_t first, uint64_t count, void* stream) {
+    return guarded([&]() -> int { umcx::run_device_noalloc(p, first, count, stream); return 0; });
+}
+
 int umcx_domain_from_json(const char* json, uint32_t dims[3], int32_t** labels) {
     return guarded([&]() -> int {
         if (!json || !dims || !labels) throw std::runtime_error("umcx_domain_from_json: null argument");

@@ -21,9 +21,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
+#include "../../include/umcx.h"
 #include "glibc_libm.h"
 #include "photon_device.h"
 #include "runtime.h"
@@ -618,6 +620,20 @@ __global__ void arith_check_kernel(int which, uint64_t n, unsigned long long* ba
     if (nbad) atomicAdd(bad, nbad);
 }
 
+// Media ids of a caller's uint8 label volume (bits 0-6; bit 7, the detector flag, is dropped)
+// packed `bits` per voxel, 32 / bits voxels per word, in the LdsLabels layout. One word per thread.
+__global__ void pack_labels_kernel(const uint8_t* lab, uint64_t nvox, uint32_t bits, uint32_t words, uint32_t* out) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    const uint32_t per = 32u / bits, mask = (1u << bits) - 1u;
+    uint32_t v = 0u;
+    for (uint32_t j = 0; j < per; j++) {
+        const uint64_t i = (uint64_t)w * per + j;
+        if (i < nvox) v |= ((uint32_t)lab[i] & 0x7Fu & mask) << (j * bits);
+    }
+    out[w] = v;
+}
+
 __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1197,6 +1213,143 @@ TallyBoxSearch device_choose_box(const float* counts, int nx, int ny, int nz, in
     memcpy(&r.in_box, &h[3], 8);
     memcpy(r.box, &h[4], sizeof r.box);
     return r;
+}
+
+// ---------------- one photon range on caller-owned buffers, with no allocation ----------------
+//
+// umcx_run_device_noalloc runs the kernel of umcx_session_run on buffers the caller owns. What a
+// session allocates beside its inputs and outputs -- the work cursor and the packed media ids
+// staged in LDS -- lives here in one file-scope device array, `scratch`, that every call on a
+// device shares (calls that overlap on one device race on it). Its address on each device comes
+// from hipGetSymbolAddress, taken once per device with that device current; host code uses only
+// that address, never the array's name.
+constexpr size_t kNoAllocCtlBytes = 256;                                  // work cursor (2 x u64), padding
+constexpr size_t kNoAllocScratchBytes = kNoAllocCtlBytes + kLabelCacheBytes;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+struct NoAllocDevice {
+    bool ready = false;
+    unsigned char* ptr = nullptr;   // this device's copy of `scratch`
+    int cus = 0;
+    size_t lds_wg_max = 0;
+};
+constexpr int kNoAllocMaxDevices = 64;
+NoAllocDevice g_noalloc[kNoAllocMaxDevices];
+std::mutex g_noalloc_mu;
+
+const NoAllocDevice& noalloc_device(int dev) {
+    if (dev < 0 || dev >= kNoAllocMaxDevices) throw std::runtime_error("umcx_run_device_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    NoAllocDevice& d = g_noalloc[dev];
+    if (!d.ready) {
+        void* ptr = nullptr;
+        HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        int cus = 0, lds = 0;
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        d.ptr = (unsigned char*)ptr;
+        d.cus = std::max(1, cus);
+        d.lds_wg_max = lds > 0 ? (size_t)lds : 64 * 1024;
+        d.ready = true;
+    }
+    return d;
+}
+}  // namespace
+
+void run_device_noalloc(const umcx_device_problem* q, uint64_t first, uint64_t count, void* stream) {
+    const char* fn = "umcx_run_device_noalloc";
+    if (!q) throw std::runtime_error(std::string(fn) + ": null problem");
+    const uint32_t nx = q->dims[0], ny = q->dims[1], nz = q->dims[2];
+    const int32_t maxgate = (int32_t)q->dims[3];
+    if (!nx || !ny || !nz || maxgate < 1) throw std::runtime_error(std::string(fn) + ": empty grid or no time gate");
+    const uint64_t nvox = (uint64_t)nx * ny * nz;
+    if (nvox >= 0x7FFFFFFFull) throw std::runtime_error(std::string(fn) + ": more than 2^31 - 1 voxels");
+    if (q->mediumnum < 1 || q->mediumnum > 16)
+        throw std::runtime_error(std::string(fn) + ": mediumnum must be 1..16 (media ids are staged at 2 or 4 bits)");
+    if (q->srctype < 0 || q->srctype >= (int32_t)kUnknownSource) throw std::runtime_error(std::string(fn) + ": unknown source type");
+    if (!q->labels || !q->media || !q->energies) throw std::runtime_error(std::string(fn) + ": null labels, media or energies");
+    if (first > ~0ull - count) throw std::runtime_error(std::string(fn) + ": photon range overflows");
+    const uint32_t bits = q->mediumnum <= 4 ? 2u : 4u;
+    const uint64_t words = (nvox * bits + 31) / 32;
+    if (words * 4 > kLabelCacheBytes)
+        throw std::runtime_error(std::string(fn) + ": the packed label volume needs " + std::to_string(words * 4) +
+                                 " bytes of scratch and at most " + std::to_string(kLabelCacheBytes) +
+                                 " are available (" + std::to_string(nvox) + " voxels at " + std::to_string(bits) +
+                                 " bits); run it through a session (umcx_session_run)");
+    const bool savevol = q->fluence != nullptr;
+    const uint64_t cells = nvox * (uint64_t)maxgate;
+    if (savevol && q->deposit_cache && cells >= 0xFFFFFFFFull)
+        throw std::runtime_error(std::string(fn) + ": the deposit cache addresses fewer than 2^32 - 1 fluence cells");
+    const bool savedet = q->issavedet != 0;
+    const int32_t detnum = savedet ? q->detnum : 0;
+    const int32_t ppathlen = (savedet && (q->savedetflag & kPPath)) ? q->mediumnum : 0;
+    const int32_t detlen = (q->savedetflag & kDetID ? 1 : 0) + ((q->savedetflag & kPPath) ? q->mediumnum : 0) +
+                           (((q->savedetflag & kExitPos) ? 1 : 0) + ((q->savedetflag & kExitDir) ? 1 : 0)) * 3;
+    if (detnum < 0) throw std::runtime_error(std::string(fn) + ": negative detector count");
+    if (savedet && (!q->ndet || !q->detid || !q->detw || (detlen > 0 && !q->detp) || (detnum > 0 && !q->detpos)))
+        throw std::runtime_error(std::string(fn) + ": detectors need detpos, detp, detid, detw and ndet");
+    if (count == 0) return;
+
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    const NoAllocDevice& d = noalloc_device(dev);
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* cursor = (unsigned long long*)d.ptr;
+    uint32_t* plabels = (uint32_t*)(d.ptr + kNoAllocCtlBytes);
+
+    // launch shape of the sessions with staged labels: one 1024-lane workgroup per CU, half-size
+    // workgroups for runs under kShortRunPhotonsPerLane photons per lane
+    int block = 1024;
+    if (count < kShortRunPhotonsPerLane * 1024ull * (uint64_t)d.cus) block = 512;
+    auto lds_of = [&](int b) {
+        return (size_t)32 * 16 + (size_t)q->mediumnum * 16 + (size_t)words * 4 +
+               (ppathlen > kRegPaths ? (size_t)ppathlen * b * 4 : 0);
+    };
+    while (lds_of(block) > d.lds_wg_max && block > 64) block /= 2;
+    if (lds_of(block) > d.lds_wg_max) throw std::runtime_error(std::string(fn) + ": media and labels exceed the LDS");
+    const int blocks = d.cus;
+
+    dev::KernelArgs ka;
+    memset(&ka, 0, sizeof ka);
+    ka.nx = nx; ka.ny = ny; ka.nz = nz; ka.nyz = ny * nz; ka.nxyz = nvox;
+    ka.maxgate = maxgate; ka.outputtype = q->outputtype; ka.savedetflag = q->savedetflag;
+    ka.mediumnum = q->mediumnum; ka.detnum = detnum; ka.srctype = q->srctype;
+    ka.issavevol = savevol ? 1 : 0; ka.ppathlen = ppathlen; ka.detlen = detlen;
+    ka.tstart = q->tstart; ka.tend = q->tend; ka.rtstep = q->rtstep;
+    ka.srcpos = make_float4(q->srcpos[0], q->srcpos[1], q->srcpos[2], q->srcpos[3]);
+    ka.srcdir = make_float4(q->srcdir[0], q->srcdir[1], q->srcdir[2], q->srcdir[3]);
+    ka.srcp1 = make_float4(q->srcparam1[0], q->srcparam1[1], q->srcparam1[2], q->srcparam1[3]);
+    ka.srcp2 = make_float4(q->srcparam2[0], q->srcparam2[1], q->srcparam2[2], q->srcparam2[3]);
+    for (int i = 0; i < 4; i++) ka.seeds[i] = q->seeds[i];
+    ka.first = first; ka.count = count;
+    ka.labels = q->labels;
+    ka.media = (const float4*)q->media;
+    ka.detpos = (const float4*)q->detpos;
+    ka.fluence = q->fluence;
+    ka.detp = q->detp; ka.detid = q->detid; ka.detw = q->detw; ka.ndet = q->ndet;
+    ka.maxdet = savedet ? q->maxdetphotons : 0u;
+    ka.energy = q->energies;
+    ka.cursor = cursor;
+    ka.dep_cache = (savevol && q->deposit_cache) ? 1 : 0;
+    ka.unit = q->unitinmm;
+    ka.plabels = plabels;
+    ka.plabel_words = (uint32_t)words;
+    ka.plabel_bits = bits;
+    ka.plabel_lg = bits == 2 ? 4u : 3u;
+    ka.plabel_id0 = 0;
+    ka.plabel_step = 1;
+    ka.per_wg = (count - count / 8) / (uint64_t)blocks;
+
+    // the cursor is cleared by a fill kernel (not a memset node, so the sequence can be captured
+    // in a graph) and the packed labels are rewritten in full by every call
+    hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, cursor, 2);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(dev::pack_labels_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, q->labels, nvox,
+                       bits, (uint32_t)words, plabels);
+    HIP_OK(hipGetLastError());
+    pick(q->isreflect != 0, savedet, true, true)(ka, blocks, block, lds_of(block), s);
+    HIP_OK(hipGetLastError());
 }
 
 void device_libm_eval(int fn, const float* in, float* out, uint64_t n, int device) {

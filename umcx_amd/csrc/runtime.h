@@ -160,3 +160,11 @@ int device_count();
 std::string device_name(int device);
 
 }  // namespace umcx
+
+struct umcx_device_problem;
+namespace umcx {
+// umcx_run_device_noalloc: the photon kernel on caller-owned device buffers, enqueued on
+// `stream` (a hipStream_t) of the current device, with no allocation (photon_kernel.hip).
+void run_device_noalloc(const umcx_device_problem* q, uint64_t first, uint64_t count, void* stream);
+
+}  // namespace umcx
