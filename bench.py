@@ -12,11 +12,15 @@ Prints one JSON line. ``roofline`` prices the photon kernel's algorithmic FP32 w
 op model of the photon loop, counted from src/umcx.cpp:315-428: 34 per step, 9 per voxel crossing,
 62 per scatter, 1 per random draw, 35 per Fresnel test) against the MI355X FP32 peak, using the
 kernel's HIP-event time on its own stream. Beside it are the two ceilings the kernel runs
-against: ``roofline.atomics`` -- memory-side float atomics per second against the rate
-umcx_amd/bin/roofline_probe measures -- and ``roofline.valu_issue``, the VALU instruction issue
-rate (instructions per photon from rocprofv3 counters) with the active-lane fraction. ``cpu_baseline`` times the reference umcx binary built
-from source (or, when it is absent, the CPU oracle) on this machine's host cores over a bounded
-sample of the same workload.
+against, each priced with per-photon rocprofv3 counters read from the kernel's committed profile
+summary (``counters.build_match`` says whether they were measured on the sources built now):
+``roofline.valu_issue`` -- busy VALU quad-cycles per second against 256 CUs x 4 SIMDs x 2.4 GHz / 4,
+the ruler the issue-rate probes of umcx_amd/bin/roofline_probe calibrate -- and
+``roofline.atomics``, memory-side float atomics per second against the rate the same probe
+measures; ``roofline.bound`` names the larger. ``cpu_baseline`` times the reference umcx binary
+built from source (or, when it is absent, the CPU oracle, and says so) on this machine's host
+cores over bounded samples of the same workload and of the small cube and the skin model, with
+the host's CPU model and glibc and the binary's provenance.
 """
 from __future__ import annotations
 
@@ -38,23 +42,52 @@ PHOTONS = 100_000_000
 COUNTS = {"S": 346.63, "X": 189.12, "K": 157.29, "R": 475.06, "F": 3.649}
 OPS_PER_PHOTON = 34 * COUNTS["S"] + 9 * COUNTS["X"] + 62 * COUNTS["K"] + COUNTS["R"] + 35 * COUNTS["F"]
 FP32_PEAK_TFLOPS = 157.3          # MI355X published FP32 (vector/matrix) peak
-# VALU issue: a wave64 FP32 instruction occupies a 16-lane SIMD for 4 cycles, so 256 CUs x 4 SIMDs
-# x 2.4 GHz / 4 = 6.14e11 wave-instructions/s (an FMA-chain probe reaches 6.03e11). The photon
-# kernel's instruction count per photon and active-lane fraction on this workload come from
-# rocprofv3 counters (profiles/photon_kernel_hetero_reflective_1e8.md: SQ_INSTS_VALU 4.701e10 and
-# SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU) = 0.402 over 1e7 photons).
-VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
-VALU_INSTR_PER_PHOTON = 4701.0
-ACTIVE_LANE_FRAC = 0.402
-# HBM traffic per photon of the same kernel (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over 1e7
-# photons: 1,106 KiB and 33,799,634 KiB, scaled by 2.0 and 1.0 -- what the two counters read for
-# a 256 MiB device-to-device copy); almost all of it is the memory-side float atomics' 32-byte
-# writes.
-HBM_BYTES_PER_PHOTON = (1106 * 2.0 + 33_799_634 * 1.0) * 1024 / 1e7
-# memory-side float atomics per photon (TCC_ATOMIC 1.081e9 over 1e7 photons) against the rate
-# umcx_amd/bin/roofline_probe measures for hashed addresses (profiles/roofline_probe_gfx950.md)
-ATOMICS_PER_PHOTON = 108.1
+# VALU issue capacity: one quad-cycle (a wave64 instruction of the 4-cycle classes) per SIMD every
+# 4 clocks, 256 CUs x 4 SIMDs at the 2.4 GHz peak engine clock = 6.144e11 busy quad-cycles/s. The
+# issue-rate probes of umcx_amd/bin/roofline_probe --valu (FP32 FMA and add, int32, compare+select,
+# move, FP64 FMA, reciprocal) each fill 98-99.5 % of it as rocprofv3 counts it: busy quad-cycles =
+# SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2 (dual-issued pairs counted once), against
+# GRBM_GUI_ACTIVE per XCD x 256 CUs (profiles/roofline_probe_gfx950.md). The kernels' busy
+# quad-cycles per photon are read from their committed counter summaries.
+VALU_BUSY_PEAK = 256 * 4 * 2.4e9 / 4
+# memory-side float atomics per second that roofline_probe measures for hashed addresses
 ATOMIC_PEAK = 2.11e10
+PHOTON_PROFILE = "profiles/photon_kernel_hetero_reflective_1e8.md"
+BATCH_PROFILE = "profiles/batch_kernel_uneven256.md"
+
+
+def counter_summary(profile: str) -> dict:
+    """The machine-readable counter block of a committed profile summary, and whether the kernel
+    sources it was measured on are the ones built now (sha256 over the listed files)."""
+    import hashlib
+    text = open(os.path.join(ROOT, profile)).read()
+    head = text.index("## Counter summary (read by bench.py)")
+    a = text.index("```json", head) + len("```json")
+    b = text.index("```", a)
+    c = json.loads(text[a:b])
+    h = hashlib.sha256()
+    for f in c["kernel_sources"]:
+        h.update(open(os.path.join(ROOT, f), "rb").read())
+    c["profile"] = profile
+    c["build_match"] = h.hexdigest() == c["kernel_sources_sha256"]
+    return c
+
+
+def ceilings(c: dict, photons: float, kernel_s: float) -> dict:
+    """VALU-issue and memory-side-atomic ceilings of a kernel from its per-photon counters."""
+    busy = c["valu_busy_quad_cycles_per_photon"] * photons / kernel_s
+    atom = c["memory_side_atomics_per_photon"] * photons / kernel_s
+    out = {"valu_issue": {"busy_quad_cycles_per_photon": c["valu_busy_quad_cycles_per_photon"],
+                          "instr_per_photon": c["valu_instr_per_photon"],
+                          "achieved_per_s": busy, "peak_per_s": VALU_BUSY_PEAK, "frac": round(busy / VALU_BUSY_PEAK, 4),
+                          "active_lane_frac": c["active_lane_frac"]},
+           "atomics": {"per_photon": c["memory_side_atomics_per_photon"], "achieved_per_s": atom,
+                       "peak_per_s": ATOMIC_PEAK, "frac": round(atom / ATOMIC_PEAK, 4)},
+           "counters": {"profile": c["profile"], "build_match": c["build_match"],
+                        "kernel_sources_sha256": c["kernel_sources_sha256"]}}
+    out["binding"] = max(("valu_issue", "atomics"), key=lambda k: out[k]["frac"])
+    return out
+
 
 CONFIG = {
     "Session": {"ID": "hetero_reflective", "Photons": PHOTONS, "RNGSeed": 1648335518, "DoMismatch": 1,
@@ -82,26 +115,91 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(target_s: float = 12.0) -> dict:
-    """Time the reference CPU build (or the oracle) on a bounded sample of the same workload."""
+def host_info() -> dict:
+    """The host the CPU baseline runs on: CPU model, sockets, cores, threads per core, glibc."""
+    info = {"threads_available": host_threads()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys and keys[k.strip()] not in info:
+                v = v.strip()
+                info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    except Exception as e:  # lscpu absent: say so
+        info["lscpu_error"] = str(e)
+    try:
+        info["glibc"] = os.confstr("CS_GNU_LIBC_VERSION")
+    except Exception:
+        info["glibc"] = None
+    return info
+
+
+def reference_provenance() -> dict | None:
+    """Source hashes, flags and compiler of oracle/_ref/umcx, written beside it by oracle/Makefile."""
+    path = os.path.join(ROOT, "oracle", "_ref", "umcx.provenance")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        k, _, v = line.strip().partition(" ")
+        if k in ("flags", "compiler"):
+            out[k] = v
+        elif v.startswith("sha256 "):
+            out[k + "_sha256"] = v.split()[1]
+    out["build"] = "oracle/Makefile target ref: g++ on the reference's src/umcx.cpp, the flags of its CPU target " \
+                   "(src/Makefile:8,64-65)"
+    return out
+
+
+def cpu_rate(cfg: dict | None, argv: list[str] | None, n: int, threads: int) -> float:
+    """photons/ms of the reference binary on n photons of a config (JSON) or a command line, from
+    its own timing window (src/umcx.cpp:812-840)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "umcx")
+    with tempfile.TemporaryDirectory() as td:
+        if cfg is not None:
+            c = json.loads(json.dumps(cfg))
+            c["Session"]["Photons"] = int(n)
+            c["Session"]["ID"] = "cpu_sample"
+            path = os.path.join(td, "in.json")
+            with open(path, "w") as f:
+                json.dump(c, f)
+            cmd = [ref, "-f", path]
+        else:
+            cmd = [ref] + list(argv) + ["-n", str(int(n))]
+        out = subprocess.run(cmd, cwd=td, capture_output=True, text=True,
+                             env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=600).stdout
+    m = LINE.search(out)
+    if not m:
+        raise RuntimeError(f"reference run printed no speed line: {out!r}")
+    return float(m.group(1))
+
+
+def bounded_sample(run, target_s: float, probe: int = 50_000) -> tuple[float, int]:
+    rate = run(probe)
+    n = int(min(max(rate * target_s * 1e3, probe), 2e8))
+    return run(n), n
+
+
+def cpu_baseline(target_s: float = 12.0, others_s: float = 6.0) -> dict:
+    """Time the reference CPU build on bounded samples of the headline workload (and of the small
+    homogeneous cube and the skin model). Without the reference binary the oracle restatement is
+    timed instead, and the result says so."""
     threads = host_threads()
     ref = os.path.join(ROOT, "oracle", "_ref", "umcx")
+    out = {"unit": "photons/ms", "cores": threads, "host": host_info()}
     if os.path.exists(ref):
-        def run(n):
-            cfg = json.loads(json.dumps(CONFIG))
-            cfg["Session"]["Photons"] = int(n)
-            cfg["Session"]["ID"] = "cpu_sample"
-            with tempfile.TemporaryDirectory() as td:
-                path = os.path.join(td, "in.json")
-                with open(path, "w") as f:
-                    json.dump(cfg, f)
-                out = subprocess.run([ref, "-f", path], cwd=td, capture_output=True, text=True,
-                                     env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=600).stdout
-            m = LINE.search(out)
-            if not m:
-                raise RuntimeError(f"reference run printed no speed line: {out!r}")
-            return float(m.group(1))
-        kind = "reference"
+        rate, n = bounded_sample(lambda k: cpu_rate(CONFIG, None, k, threads), target_s)
+        out.update(value=round(rate, 2), kind="reference", reference=reference_provenance(),
+                   sample=f"{n} photons of the same workload (fluence and detectors on), {threads} OpenMP threads; "
+                          "rate from the binary's own timing window")
+        others = {}
+        for name, cfg, argv in (("cube60_1e6", None, ["--bench", "cube60", "-d", "0"]),
+                                ("skinvessel_200cube_1e8", SKIN_CONFIG, None)):
+            r, k = bounded_sample(lambda k: cpu_rate(cfg, argv, k, threads), others_s, probe=20_000)
+            others[name] = {"value": round(r, 2), "photons": k}
+        out["other_configs"] = others
     else:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O  # the checker, used here only as the timed CPU baseline
@@ -109,18 +207,15 @@ def cpu_baseline(target_s: float = 12.0) -> dict:
         import numpy as np
         prob = O.problem_from_config(CONFIG, np.asarray(O_domain(), dtype=np.int32))
 
-        def run(n):
+        def run(k):
             t0 = time.perf_counter()
-            O.run(prob, nphoton=int(n), nthreads=threads)
-            return n / ((time.perf_counter() - t0) * 1e3)
-        kind = "port"
-    probe = 50_000
-    rate = run(probe)
-    n = int(min(max(rate * target_s * 1e3, probe), 2e8))
-    rate = run(n)
-    return {"value": round(rate, 2), "unit": "photons/ms", "cores": threads, "kind": kind,
-            "sample": f"{n} photons of the same workload (fluence and detectors on), {threads} OpenMP threads; "
-                      "rate from the binary's own timing window"}
+            O.run(prob, nphoton=int(k), nthreads=threads)
+            return k / ((time.perf_counter() - t0) * 1e3)
+        rate, n = bounded_sample(run, target_s)
+        out.update(value=round(rate, 2), kind="port",
+                   note="reference binary oracle/_ref/umcx absent: the oracle's C restatement was timed instead",
+                   sample=f"{n} photons of the same workload through the oracle, {threads} OpenMP threads")
+    return out
 
 
 def batched_leg(sync) -> dict:
@@ -180,35 +275,63 @@ def batched_leg(sync) -> dict:
         "noise_rel_std_rms": float(np.sqrt(np.mean(rel ** 2))), "noise_rel_std_max_est": float(rel.max()),
         "batch_vs_plain_max_rel_diff": float(np.max(np.abs(roi_b - roi_p) / roi_p)),
         "at_1pct_noise": matched,
+        "roofline": ceilings(counter_summary(BATCH_PROFILE), photons, batch_ms * 1e-3),
     }
+
+
+def replica_noise(cfgs, shape, reps: int, seed_base: int):
+    """Relative standard deviation of each job's reported quantity from `reps` independent
+    replicas (seeds changed per replica), all run in one batch."""
+    import copy
+
+    import numpy as np
+
+    from umcx_amd import batch as B
+    rep_jobs = []
+    for c in cfgs:
+        for r in range(reps):
+            c2 = copy.deepcopy(c)
+            c2["Session"]["RNGSeed"] = int(c["Session"]["RNGSeed"]) + seed_base + 1000 * (r + 1)
+            rep_jobs.append(c2)
+    with B.BatchSimulation(rep_jobs, **shape) as bs:
+        bs.run()
+        roi = np.array([B.roi_sum(bs.fluence(j), rep_jobs[j]) for j in range(len(rep_jobs))]).reshape(len(cfgs), reps)
+    return roi.std(axis=1, ddof=1) / roi.mean(axis=1)
 
 
 def matched_noise_batch(jobs, chosen, sync, target: float = 0.01, reps: int = 8, margin: float = 3.0) -> dict:
     """The same comparison at the photon count each job needs for a 1 % relative standard
-    deviation of its reported quantity. Per-job noise comes from `reps` independent replicas of
-    N/reps photons; the count is margin x N (s/target)^2, at least 2000 photons and at most N;
-    the noise reached is checked with a second batch at changed seeds."""
+    deviation of its reported quantity. A first estimate comes from `reps` replicas of N/reps
+    photons (count = margin x N (s/target)^2, at least 2000); then, at the chosen counts, `reps`
+    independent replicas of every job measure its noise, and any job above the target gets
+    1.2 x (s/target)^2 times its count and is measured again (at most three raises)."""
     import copy
 
     import numpy as np
 
     from umcx_amd import batch as B
     shape = dict(block=chosen["block"], blocks=chosen["blocks"], tally_mode=1 if chosen["tile_cells"] else 0)
-    rep_jobs = []
+    first = []
     for c in jobs:
-        for r in range(reps):
-            c2 = copy.deepcopy(c)
-            c2["Session"]["Photons"] = max(1, int(c["Session"]["Photons"]) // reps)
-            c2["Session"]["RNGSeed"] = int(c["Session"]["RNGSeed"]) + 1000 * (r + 1)
-            rep_jobs.append(c2)
-    with B.BatchSimulation(rep_jobs, **shape) as bs:
-        bs.run()
-        roi = np.array([B.roi_sum(bs.fluence(j), rep_jobs[j]) for j in range(len(rep_jobs))]).reshape(len(jobs), reps)
-    rel_full = roi.std(axis=1, ddof=1) / np.sqrt(reps) / roi.mean(axis=1)      # at the full count N
+        c2 = copy.deepcopy(c)
+        c2["Session"]["Photons"] = max(1, int(c["Session"]["Photons"]) // reps)
+        first.append(c2)
+    rel_full = replica_noise(first, shape, reps, 0) / np.sqrt(reps)             # at the full count N
     small = copy.deepcopy(jobs)
     for c, s_ in zip(small, rel_full):
         n = int(c["Session"]["Photons"])
-        c["Session"]["Photons"] = int(min(n, max(2000, np.ceil(margin * n * (s_ / target) ** 2))))
+        c["Session"]["Photons"] = int(max(2000, np.ceil(margin * n * (s_ / target) ** 2)))
+    raised, rounds = set(), 0
+    for rounds in range(1, 5):
+        rel = replica_noise(small, shape, reps, 100000 * rounds)
+        over = np.flatnonzero(rel > target)
+        log(f"bench: 1% noise round {rounds}: max {rel.max():.4f}, {len(over)} jobs above")
+        if len(over) == 0 or rounds == 4:
+            break
+        for j in over:
+            c = small[j]["Session"]
+            c["Photons"] = int(np.ceil(int(c["Photons"]) * 1.2 * (rel[j] / target) ** 2))
+            raised.add(int(j))
     photons = sum(int(c["Session"]["Photons"]) for c in small)
     t0 = time.perf_counter()
     with B.BatchSimulation(small, **shape) as bs:
@@ -218,14 +341,6 @@ def matched_noise_batch(jobs, chosen, sync, target: float = 0.01, reps: int = 8,
         t0 = time.perf_counter()
         batch_ms = bs.run()
         batch_wall = (time.perf_counter() - t0) * 1e3
-        roi_a = np.array([B.roi_sum(bs.fluence(j), small[j]) for j in range(len(small))])
-    small2 = copy.deepcopy(small)
-    for c in small2:
-        c["Session"]["RNGSeed"] = int(c["Session"]["RNGSeed"]) + 1
-    with B.BatchSimulation(small2, **shape) as bs:
-        bs.run()
-        roi_b = np.array([B.roi_sum(bs.fluence(j), small2[j]) for j in range(len(small2))])
-    rel = np.abs(roi_a - roi_b) / np.sqrt(2.0) / (0.5 * (roi_a + roi_b))
     log(f"bench: plain loop at the 1% noise counts ({photons:.3e} photons)")
     _, loop_kernel_ms, loop_wall_ms = B.plain_loop(small, keep=False)
     return {
@@ -237,7 +352,11 @@ def matched_noise_batch(jobs, chosen, sync, target: float = 0.01, reps: int = 8,
         "speedup_vs_plain_wall": round(loop_wall_ms / batch_wall, 3),
         "setup_ms": round(setup_ms, 2),
         "speedup_vs_plain_wall_incl_setup": round(loop_wall_ms / (batch_wall + setup_ms), 3),
-        "noise_rel_std_rms": float(np.sqrt(np.mean(rel ** 2))), "noise_rel_std_max_est": float(rel.max()),
+        "noise_replicas": reps, "noise_rounds": rounds, "jobs_raised": sorted(raised),
+        "noise_rel_std_max": float(rel.max()), "noise_rel_std_rms": float(np.sqrt(np.mean(rel ** 2))),
+        "noise_rel_std_median": float(np.median(rel)),
+        "noise_estimator": f"per job, sample std / mean of the reported quantity over {reps} independent replicas "
+                           "at the final counts (the last round's measurement)",
     }
 
 
@@ -282,13 +401,8 @@ def jacobian_at(sync, photons: int, fd_voxels: int) -> dict:
     jac_wall = (time.perf_counter() - t0) * 1e3
     S, D, nvox = len(srcs), m.ndet, m.nvox
     entries = S * D * nvox * 2
-    I = m.I.copy()
-    w2 = np.zeros_like(I)
-    for s_, (ids, det, w) in enumerate(m.records):
-        np.add.at(w2[s_], det, w.astype(np.float64) ** 2)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        noise = np.sqrt(w2) / I
-    recs = sum(len(r[0]) for r in m.records)
+    noise = m.noise()
+    recs = m.nrecords
     log(f"bench: jacobian forward {m.forward_ms:.1f} ms, replay {m.replay_ms:.1f} ms, {recs} records")
     # the inclusion's voxels carry the largest mua sensitivity; FD columns of source 0 there
     lab = np.asarray(__import__("umcx_amd").domain(cfg) & 0x7FFFFFFF).reshape(-1)
@@ -397,6 +511,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-plain leg")
     ap.add_argument("--no-jacobian", action="store_true", help="skip the Jacobian-vs-plain leg")
     ap.add_argument("--no-others", action="store_true", help="skip the other configurations")
+    ap.add_argument("--batch-profile", nargs="?", const="batch", choices=["batch", "plain"],
+                    help="only build the 256-job batch (autotuned) and run it once, or ('plain') run the loop of "
+                         "plain launches over the same jobs once (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -409,6 +526,18 @@ def main():
         sync = torch.cuda.synchronize
     except Exception:
         torch, sync = None, (lambda: None)
+    if args.batch_profile:
+        from umcx_amd import batch as B
+        jobs = B.uneven_jobs(seed=7, njobs=256)
+        photons = sum(int(j["Session"]["Photons"]) for j in jobs)
+        if args.batch_profile == "plain":
+            _, kms, wall = B.plain_loop(jobs, keep=False)
+            print(json.dumps({"photons": photons, "plain_loop_kernel_ms": kms, "plain_loop_wall_ms": wall}))
+            return
+        with B.BatchSimulation(jobs) as bs:
+            ms = bs.run()
+        print(json.dumps({"batched_photons": photons, "batch_ms": ms}))
+        return
 
     cfg = json.loads(json.dumps(CONFIG))
     cfg["Session"]["Photons"] = args.photons
@@ -447,6 +576,16 @@ def main():
     value = args.photons * args.steps / wall_ms
     kavg = sum(kernel_ms) / len(kernel_ms)
     achieved_tflops = OPS_PER_PHOTON * args.photons / (kavg * 1e-3) / 1e12
+    # the kernel's per-photon counters come from its committed profile summary (build_match says
+    # whether they were measured on the sources built now)
+    pc = counter_summary(PHOTON_PROFILE)
+    ceil = ceilings(pc, args.photons, kavg * 1e-3)
+    roofline = {"bound": "valu" if ceil["binding"] == "valu_issue" else "atomics",
+                "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
+                "traffic": round(pc["hbm_bytes_per_photon"] * args.photons),
+                "traffic_gbps": round(pc["hbm_bytes_per_photon"] * args.photons / (kavg * 1e-3) / 1e9, 1),
+                "ops_per_photon": round(OPS_PER_PHOTON, 1), "kernel_ms": round(kavg, 3), **ceil}
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "photons/ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
@@ -454,20 +593,7 @@ def main():
         "config": {"workload": "heterogeneous reflective: 60^3 1 mm voxels, sphere r=15 (n=1.0) in n=1.37 medium, "
                                "Fresnel on, pencil (29.5,29.5,0)+z, 50 gates x 0.1 ns, 4 disk detectors r=1",
                    "photons_per_step": args.photons, "seed": 1648335518},
-        "roofline": {"bound": "valu", "binding": "atomics", "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 5),
-                     "traffic": round(HBM_BYTES_PER_PHOTON * args.photons),
-                     "traffic_gbps": round(HBM_BYTES_PER_PHOTON * args.photons / (kavg * 1e-3) / 1e9, 1),
-                     "ops_per_photon": round(OPS_PER_PHOTON, 1), "kernel_ms": round(kavg, 3),
-                     "valu_issue": {"instr_per_photon": VALU_INSTR_PER_PHOTON,
-                                    "achieved_per_s": VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3),
-                                    "peak_per_s": VALU_ISSUE_PEAK,
-                                    "frac": round(VALU_INSTR_PER_PHOTON * args.photons / (kavg * 1e-3) / VALU_ISSUE_PEAK, 4),
-                                    "active_lane_frac": ACTIVE_LANE_FRAC},
-                     "atomics": {"per_photon": ATOMICS_PER_PHOTON,
-                                 "achieved_per_s": ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3),
-                                 "peak_per_s": ATOMIC_PEAK,
-                                 "frac": round(ATOMICS_PER_PHOTON * args.photons / (kavg * 1e-3) / ATOMIC_PEAK, 4)}},
+        "roofline": roofline,
         "end_to_end": {"wall_ms": round(e2e_ms, 1), "photons_per_ms": round(args.photons / e2e_ms, 1),
                        "includes": "session set-up, label upload, counting pilot, photon loop, read-back of the "
                                    "fluence grid and the detected rows over PCIe"},
@@ -487,7 +613,14 @@ def main():
     if not args.no_cpu:
         log("bench: CPU baseline")
         try:
-            out["cpu_baseline"] = cpu_baseline()
+            cb = cpu_baseline()
+            ratios = {"hetero_reflective_1e8": round(value / cb["value"], 1)}
+            for name, v in cb.get("other_configs", {}).items():
+                gpu = out.get("other_configs", {}).get(name)
+                if gpu:
+                    ratios[name] = round(gpu["photons_per_ms"] / v["value"], 1)
+            cb["gpu_over_cpu"] = ratios
+            out["cpu_baseline"] = cb
         except Exception as e:  # report, never fake
             out["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(out), flush=True)

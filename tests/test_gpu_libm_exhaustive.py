@@ -7,8 +7,9 @@ ranges the photon loop can reach, and beyond:
 * expf: every float from -inf to +inf (the loop takes expf(-mua d) <= 0, :326);
 * acosf: every float of [-1, 1] (Henyey-Greenstein cosines for any g, 2u - 1, cone angles:
   :232, :372, :374);
-* sinf and cosf: every float with |y| < 120, the domain of the replicas' quadrant reduction
-  (2 pi u, scattering and launch angles: :437-438).
+* sinf and cosf: every float (2 pi u and scattering angles, :437-438, stay below 120, where the
+  replicas reduce in double; user-set cone and fourier angles, :231, :252, may reach any value,
+  where the reduction uses the bits of 2/pi in integer arithmetic).
 
 Results compare as bit patterns; two NaNs count as equal (the two processors' default NaNs differ
 in sign). Inputs are streamed in chunks of 2^27 floats.
@@ -77,6 +78,6 @@ def test_acosf_every_float_of_the_unit_interval():
 
 
 @pytest.mark.parametrize("fn,name", [(1, "sinf"), (2, "cosf")])
-def test_sin_cos_every_float_below_120(fn, name):
-    top = 0x42F00000 - 1          # largest float below 120
-    run_ranges(name, fn, [(0x00000000, top), (0x80000000, 0x80000000 | top)])
+def test_sin_cos_every_float(fn, name):
+    # every finite float and the infinities (NaN inputs excluded)
+    run_ranges(name, fn, [(0x00000000, 0x7F800000), (0x80000000, 0xFF800000)])

@@ -84,6 +84,32 @@ def test_per_photon_state_matches_oracle(name, label_cache):
     assert np.array_equal(exact(t["len"][:, 3]), exact(o["len"][:, 3]))
 
 
+@pytest.mark.parametrize("name,param,value", [("fourier", ("Param1", 3), 37.3), ("fourier", ("Param2", 3), 41.6),
+                                              ("cone", ("Param1", 0), 200.5)])
+def test_source_angles_past_120_match_oracle(name, param, value):
+    """User-set source angles reach sinf/cosf arguments of 120 and more, where the replicas take
+    the large-argument reduction: fourier phases (2 pi (kx u + ky v + frac)) with kx or ky near 40
+    and a cone half-angle of 200.5 rad; per-photon states, launch weights included, bitwise equal
+    to the oracle's."""
+    import copy
+    g, _ = O.golden_problem(name)
+    cfg = copy.deepcopy(g["cfg"])
+    cfg["Optode"]["Source"][param[0]][param[1]] = value
+    prob = O.problem_from_config(cfg, U.domain(cfg))
+    n = 3000
+    say(f"trace parity {name} {param} = {value}")
+    o = O.run(prob, nphoton=n, traces=True, nthreads=8)["traces"]
+    with U.Simulation(cfg, trace=True) as sim:
+        sim.run(0, n)
+        t = sim.trace(n)
+    exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
+    for key in ("pos", "vec", "len", "w0"):
+        assert np.array_equal(exact(t[key]), exact(o[key])), key
+    assert np.array_equal(t["lastvoxelidx"], o["lastvoxelidx"])
+    if name == "fourier":                     # the launch weights spread over (0, 1]: the phase matters
+        assert len(np.unique(t["w0"])) > 100
+
+
 @pytest.mark.parametrize("label_cache", [0, -1])
 @pytest.mark.parametrize("name", DETP_CASES)
 def test_detected_rows_match_reference(name, label_cache):

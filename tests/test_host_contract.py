@@ -119,6 +119,18 @@ def test_libm_replicas_match_host_libm_on_reachable_inputs():
         assert np.array_equal(U.libm_eval(fn, th).view(np.uint32), O.host_libm(fn, th).view(np.uint32))
 
 
+def test_sin_cos_replicas_on_a_sample_of_every_float():
+    """Every 1009th bit pattern of all finite floats (both signs, arguments of 120 and more
+    included, where the large-argument reduction runs) against the host library; the GPU test
+    covers every float."""
+    for lo, hi in ((0x00000000, 0x7F800000), (0x80000000, 0xFF800000)):
+        x = np.arange(lo, hi + 1, 1009, dtype=np.uint64).astype(np.uint32).view(np.float32)
+        for fn in (1, 2):
+            mine, ref = U.libm_eval(fn, x), O.host_libm(fn, x)
+            diff = (mine.view(np.uint32) != ref.view(np.uint32)) & ~(np.isnan(mine) & np.isnan(ref))
+            assert not diff.any(), (fn, x[diff][:3])
+
+
 def test_replica_check_has_teeth():
     """(float)log(double) differs from glibc's logf on ~61k reachable inputs; the replica must not."""
     x = O.reachable_inputs()[0]

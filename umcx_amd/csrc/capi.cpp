@@ -490,7 +490,7 @@ int umcx_libm_host(int32_t fn, const float* in, float* out, uint64_t n) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < (int64_t)n; i++) {
         float x = in[i];
-        out[i] = fn == 0 ? L::logf_host(x) : fn == 1 ? L::sinf_host(x) : fn == 2 ? L::cosf_host(x)
+        out[i] = fn == 0 ? L::logf_host(x) : fn == 1 ? L::sinf_any(x) : fn == 2 ? L::cosf_any(x)
                  : fn == 3 ? L::acosf_host(x) : L::expf_host(x);
     }
     return 0;

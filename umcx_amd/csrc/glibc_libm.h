@@ -18,8 +18,10 @@
 // and expf [-104,88] (0 mismatches), and on the GPU over every argument the photon loop can form
 // (tests/test_host_contract.py, tests/test_gpu_parity.py).
 //
-// Only the argument ranges the photon loop uses are implemented: logf for positive normal
-// inputs (u + FLT_EPSILON >= 2^-23), sinf/cosf for |x| < 120, acosf on [-1, 1].
+// Implemented ranges: logf for positive normal inputs (the loop's u + FLT_EPSILON >= 2^-23),
+// sinf/cosf for every argument (sincosf_host, the fast form for |x| < 120 used by scattering;
+// sincosf_any for all floats, used where a user-set angle enters), acosf on [-1, 1], expf for
+// every float.
 #pragma once
 #include <stdint.h>
 
@@ -148,6 +150,66 @@ UMCX_HD void sincosf_host(float y, float* s, float* c) {
 }
 UMCX_HD float sinf_host(float y) { float s, c; sincosf_host(y, &s, &c); return s; }
 UMCX_HD float cosf_host(float y) { float s, c; sincosf_host(y, &s, &c); return c; }
+
+// The binary fraction of 2/pi in 32-bit windows: entry i >= 3 holds fraction bits
+// [8(i-3), 8(i-3)+32); entries 0-2 hold its first 8, 16 and 24 bits. Computed from pi (Machin's
+// formula in integer arithmetic), not copied from any binary.
+#if defined(__HIPCC__)
+__device__ __constant__
+#endif
+static const uint32_t kInvPio4[24] = {
+    0x000000a2u, 0x0000a2f9u, 0x00a2f983u, 0xa2f9836eu, 0xf9836e4eu, 0x836e4e44u, 0x6e4e4415u, 0x4e441529u,
+    0x441529fcu, 0x1529fc27u, 0x29fc2757u, 0xfc2757d1u, 0x2757d1f5u, 0x57d1f534u, 0xd1f534ddu, 0xf534ddc0u,
+    0x34ddc0dbu, 0xddc0db62u, 0xc0db6295u, 0xdb629599u, 0x6295993cu, 0x95993c43u, 0x993c4390u, 0x3c439041u,
+};
+
+// Reduction of |y| >= 120 by pi/2, in integers: the 24-bit mantissa (shifted by the low three
+// exponent bits) times the 96 bits of 2/pi that matter at this exponent gives the fraction of
+// |y| 2/pi in 2.62 fixed point. Returns r in [-pi/4, pi/4] (accurate to about 33 bits) and the
+// quadrant in *np. Only the low 32 bits of the first product are kept.
+UMCX_HD double reduce_large(uint32_t xi, int* np, const uint32_t* tab) {
+    const double pi63 = 0x1.921FB54442D18p-62;
+    const uint32_t* arr = tab + ((xi >> 26) & 15u);
+    const int shift = (int)((xi >> 23) & 7u);
+    const uint32_t m = ((xi & 0xffffffu) | 0x800000u) << shift;
+    uint64_t res0 = (uint64_t)(uint32_t)(m * arr[0]);
+    const uint64_t res1 = (uint64_t)m * arr[4];
+    const uint64_t res2 = (uint64_t)m * arr[8];
+    res0 = (res2 >> 32) | (res0 << 32);
+    res0 += res1;
+    const uint64_t n = (res0 + (1ull << 61)) >> 62;
+    res0 -= n << 62;
+    *np = (int)n;
+    return (double)(int64_t)res0 * pi63;
+}
+
+// sinf and cosf of any argument, each bitwise equal to the host library: |y| < 120 as
+// sincosf_host; larger finite arguments through reduce_large, where the argument's sign enters
+// the quadrant as (n + sign); infinities and NaN give NaN. For source sampling, whose angles the
+// user sets (cone half-angle, fourier phase).
+UMCX_HD void sincosf_any(float y, float* s, float* c, const uint32_t* tab) {
+    const uint32_t top = abstop12(y);
+    if (top < abstop12(120.0f)) {
+        sincosf_host(y, s, c);
+        return;
+    }
+    if (top >= 0x7f8u) {
+        *s = *c = (y - y) / (y - y);
+        return;
+    }
+    const uint32_t xi = f2u(y);
+    int n = 0;
+    const double r = reduce_large(xi, &n, tab);
+    const int q = (n + (int)(xi >> 31)) & 3;
+    const double sg = (q == 1 || q == 2) ? -1.0 : 1.0;
+    const double r2 = r * r;
+    const float S = sin_poly(r * sg, r2), C = cos_poly(r2);
+    const float Cq = (q & 2) ? -C : C;
+    *s = (n & 1) ? Cq : S;
+    *c = (n & 1) ? S : Cq;
+}
+UMCX_HD float sinf_any(float y) { float s, c; sincosf_any(y, &s, &c, kInvPio4); return s; }
+UMCX_HD float cosf_any(float y) { float s, c; sincosf_any(y, &s, &c, kInvPio4); return c; }
 
 UMCX_HD float sqrtf_cr(float x) {
 #if defined(__HIPCC__)

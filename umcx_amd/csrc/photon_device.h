@@ -152,7 +152,7 @@ __device__ __forceinline__ float launch(Photon& p, Rng& r, const Source& a) {
         if (st == kIsotropic) {
             th = libm::acosf_host(2.f * rng_u01(r) - 1.f);
         } else {
-            float ca = libm::cosf_host(a.srcp1.x);
+            float ca = libm::cosf_any(a.srcp1.x);   // a user-set half-angle: any float
             th = libm::acosf_host(rng_u01(r) * (1.0f - ca) + ca);
         }
         libm::sincosf_host(th, &sth, &cth);
@@ -181,7 +181,7 @@ __device__ __forceinline__ float launch(Photon& p, Rng& r, const Source& a) {
         if (st == kFourier) {
             float f1 = floorf(a.srcp1.w), f2 = floorf(a.srcp2.w);
             float arg = (f1 * u + f2 * v + a.srcp1.w - f1) * kPi * 2.f;
-            p.pw *= (libm::cosf_host(arg) * (1.f - a.srcp2.w + f2) + 1.f) * 0.5f;
+            p.pw *= (libm::cosf_any(arg) * (1.f - a.srcp2.w + f2) + 1.f) * 0.5f;   // user-set frequencies
         }
     }
     p.rx = 1.f / a.srcdir.x;   // from the nominal direction (:255)

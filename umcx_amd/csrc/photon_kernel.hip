@@ -640,8 +640,8 @@ __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n
     float x = in[i], y;
     switch (fn) {
         case 0: y = libm::logf_host(x); break;
-        case 1: y = libm::sinf_host(x); break;
-        case 2: y = libm::cosf_host(x); break;
+        case 1: y = libm::sinf_any(x); break;
+        case 2: y = libm::cosf_any(x); break;
         case 3: y = libm::acosf_host(x); break;
         default: y = libm::expf_host(x); break;
     }

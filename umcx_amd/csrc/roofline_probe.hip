@@ -7,12 +7,17 @@
 //            one hot 256-float region (same-address serialisation at the memory side), and the
 //            hashed rate over footprints from 64 KiB to 1 GiB, and with the nt / sc1 cache-policy bits.
 //
-// Prints one JSON line. Every index is taken modulo the buffer size; the launches are bounded.
+//   --valu:  issue rates of the instruction classes the photon loop mixes (FP32 FMA and add,
+//            32-bit integer, compare + select, register move, FP64 FMA, transcendental), each
+//            from one probe kernel, in wave instructions per second.
+//
+// Prints JSON lines. Every index is taken modulo the buffer size; the launches are bounded.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #define CK(x)                                                                                  \
     do {                                                                                       \
@@ -34,6 +39,60 @@ __global__ void __launch_bounds__(256) fma_chains(float* out, int iters, float a
             x6 = __builtin_fmaf(x6, a, b); x7 = __builtin_fmaf(x7, a, b);
         }
     }
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+
+// Issue-rate probes, one instruction class each, for the instruction mix of the photon loop. Each
+// lane runs 8 independent chains through 16 unrolled steps per iteration, so every step is 8
+// instructions of the class (the loop counter runs on the scalar unit). The class of each loop
+// body is checked in the gfx950 assembly (make -C umcx_amd/csrc probe_asm).
+#define PROBE_BODY(OP)                                                                          \
+    for (int i = 0; i < iters; i++) {                                                           \
+        _Pragma("unroll") for (int k = 0; k < 16; k++) {                                        \
+            OP(x0); OP(x1); OP(x2); OP(x3); OP(x4); OP(x5); OP(x6); OP(x7);                     \
+        }                                                                                       \
+    }
+
+// v_add_f32
+__global__ void __launch_bounds__(256) add_f32_chains(float* out, int iters, float a) {
+    float x0 = threadIdx.x, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+#define OP_ADD(x) x = x + a
+    PROBE_BODY(OP_ADD)
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+// v_add_u32 and v_xor_b32, alternating
+__global__ void __launch_bounds__(256) int32_chains(uint32_t* out, int iters, uint32_t a, uint32_t b) {
+    uint32_t x0 = threadIdx.x, x1 = x0 * 3, x2 = x0 * 5, x3 = x0 * 7, x4 = x0 * 11, x5 = x0 * 13, x6 = x0 * 17, x7 = x0 * 19;
+#define OP_INT(x) x = (x + a) ^ b
+    PROBE_BODY(OP_INT)
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 ^ x1 ^ x2 ^ x3 ^ x4 ^ x5 ^ x6 ^ x7;
+}
+// v_cmp_gt_f32 + v_cndmask_b32: a compare into a lane mask and a select on it
+__global__ void __launch_bounds__(256) select_chains(float* out, int iters, float a, float b) {
+    float x0 = threadIdx.x, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+#define OP_SEL(x) x = (x > a) ? b : x
+    PROBE_BODY(OP_SEL)
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+// v_mov_b32 (register copies, as value selects and the deposit cache's shifts produce)
+__global__ void __launch_bounds__(256) mov_chains(float* out, int iters) {
+    float x0 = threadIdx.x, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+#define OP_MOV(x) asm volatile("v_mov_b32 %0, %0" : "+v"(x))
+    PROBE_BODY(OP_MOV)
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+// v_fma_f64 (the double-precision evaluation of the libm replicas)
+__global__ void __launch_bounds__(256) fma_f64_chains(double* out, int iters, double a, double b) {
+    double x0 = threadIdx.x, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+#define OP_F64(x) x = __builtin_fma(x, a, b)
+    PROBE_BODY(OP_F64)
+    out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
+}
+// v_rcp_f32 (the transcendental unit: reciprocal and square-root estimates)
+__global__ void __launch_bounds__(256) trans_f32_chains(float* out, int iters) {
+    float x0 = threadIdx.x + 1, x1 = x0 + 1, x2 = x0 + 2, x3 = x0 + 3, x4 = x0 + 4, x5 = x0 + 5, x6 = x0 + 6, x7 = x0 + 7;
+#define OP_RCP(x) x = __builtin_amdgcn_rcpf(x)
+    PROBE_BODY(OP_RCP)
     out[blockIdx.x * blockDim.x + threadIdx.x] = x0 + x1 + x2 + x3 + x4 + x5 + x6 + x7;
 }
 
@@ -59,7 +118,54 @@ __global__ void __launch_bounds__(256) spread_atomics_policy(float* buf, uint32_
     }
 }
 
-int main() {
+// Times each issue-rate probe: wave64 instructions of its class per second (the counters of the
+// same dispatches, SQ_INSTS_VALU and its per-class parts, check the count).
+static int run_valu_probes() {
+    hipEvent_t e0, e1;
+    CK(hipEventCreate(&e0));
+    CK(hipEventCreate(&e1));
+    const int blocks = 16384, threads = 256, iters = 1000;
+    void* out;
+    CK(hipMalloc(&out, (size_t)blocks * threads * 8));
+    const double winstr = (double)blocks * threads / 64 * iters * 128;
+    struct Probe {
+        const char* name;
+        int per_op;   // wave instructions of the class per chain step
+        void (*launch)(void*, int, int, int);
+    };
+    const Probe probes[] = {
+        {"fma_f32", 1, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(fma_chains, dim3(b), dim3(t), 0, 0, (float*)o, it, 0.999f, 0.001f); }},
+        {"add_f32", 1, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(add_f32_chains, dim3(b), dim3(t), 0, 0, (float*)o, it, 0.5f); }},
+        {"int32_add_xor", 2, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(int32_chains, dim3(b), dim3(t), 0, 0, (uint32_t*)o, it, 0x9E3779B9u, 0x85EBCA6Bu); }},
+        {"cmp_cndmask", 2, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(select_chains, dim3(b), dim3(t), 0, 0, (float*)o, it, 1e30f, 0.5f); }},
+        {"mov_b32", 1, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(mov_chains, dim3(b), dim3(t), 0, 0, (float*)o, it); }},
+        {"fma_f64", 1, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(fma_f64_chains, dim3(b), dim3(t), 0, 0, (double*)o, it, 0.999, 0.001); }},
+        {"rcp_f32", 1, [](void* o, int b, int t, int it) { hipLaunchKernelGGL(trans_f32_chains, dim3(b), dim3(t), 0, 0, (float*)o, it); }},
+    };
+    printf("{\"valu_probes\": {");
+    bool first = true;
+    for (const Probe& p : probes) {
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; rep++) {            // the second launch is timed
+            CK(hipEventRecord(e0));
+            p.launch(out, blocks, threads, iters);
+            CK(hipGetLastError());
+            CK(hipEventRecord(e1));
+            CK(hipEventSynchronize(e1));
+            CK(hipEventElapsedTime(&ms, e0, e1));
+        }
+        const double n = winstr * p.per_op;
+        printf("%s\"%s\": {\"ms\": %.4f, \"wave_instr\": %.6e, \"wave_instr_per_s\": %.4e}", first ? "" : ", ", p.name,
+               ms, n, n / (ms * 1e-3));
+        first = false;
+    }
+    printf("}}\n");
+    CK(hipFree(out));
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1 && std::string(argv[1]) == "--valu") return run_valu_probes();
     hipEvent_t e0, e1;
     CK(hipEventCreate(&e0));
     CK(hipEventCreate(&e1));

@@ -34,6 +34,24 @@ def _torch():
     return torch
 
 
+_HIP = None
+
+
+def _device_copy(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device hipMemcpy through PyTorch's HIP runtime (the one libumcx.so runs on)."""
+    global _HIP
+    if _HIP is None:
+        import os
+        torch = _torch()
+        h = C.CDLL(os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so"))
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipMemcpy.restype = C.c_int
+        _HIP = h
+    rc = _HIP.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy device to device failed ({rc})")
+
+
 class MultiSourceModel:
     """One volume, several pencil sources and the configuration's detectors.
 
@@ -58,7 +76,9 @@ class MultiSourceModel:
         self.shape = (nx, ny, nz)
         self.nvox = nx * ny * nz
         self.ndet = int(self.sim.info["detnum"])
-        self.records = []          # per source: (ids int32, det int64, w float32) host arrays
+        self._records = None       # per source host arrays, built from the device records on use
+        self._dev = None           # device records of the last forward (ids, source, pair, weight, I)
+        self.nrecords = 0
         self.I = None              # (S, D) float64
 
     # -- forward ---------------------------------------------------------------------------------
@@ -80,11 +100,12 @@ class MultiSourceModel:
             check(L.umcx_session_set_voxel_props(self.sim._h, mua.ctypes.data, mus.ctypes.data, 0))
 
     def forward(self, mua=None, mus=None) -> np.ndarray:
-        """Simulate every source; returns log I as a (sources, detectors) float64 array."""
+        """Simulate every source; returns log I as a (sources, detectors) float64 array. The
+        detection records stay on the GPU: they are copied device to device out of the session's
+        buffers, and I_sd, the record slices and the coefficients are formed there."""
+        torch = _torch()
         self._set_props(mua, mus)
-        self.records = []
-        S, N = len(self.sources), self.nphoton
-        I = np.zeros((S, self.ndet), dtype=np.float64)
+        S, N, D = len(self.sources), self.nphoton, self.ndet
         L = lib()
         # every source in one launch: photon slot p is photon p % N of source p // N
         table = np.array([list(p) + list(d) for p, d in self.sources], dtype=np.float32).reshape(-1)
@@ -95,19 +116,53 @@ class MultiSourceModel:
         detected, saved = self.sim.detected()
         if saved < detected:
             raise RuntimeError(f"{detected - saved} detected photons did not fit MaxDetPhoton; raise max_records")
-        rows, slots, w = self.sim.detp()
-        det = rows[:, 0].astype(np.int64) - 1
-        src = slots.astype(np.int64) // N
-        order = np.argsort(src, kind="stable")
-        src, slots, det, w = src[order], slots[order], det[order], w[order]
-        bounds = np.searchsorted(src, np.arange(S + 1))
-        for s in range(S):
-            a, b = bounds[s], bounds[s + 1]
-            np.add.at(I[s], det[a:b], w[a:b].astype(np.float64))
-            self.records.append(((slots[a:b] - s * N).astype(np.int32), det[a:b], w[a:b]))
-        self.I = I
+        dev = f"cuda:{self.device}"
+        cols = int(self.sim.info["detp_columns"])
+        d_ids, d_w, d_rows = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(L.umcx_session_detp_device(self.sim._h, C.byref(d_ids), C.byref(d_w), C.byref(d_rows)))
+        slots = torch.empty(max(saved, 1), dtype=torch.int32, device=dev)
+        w = torch.empty(max(saved, 1), dtype=torch.float32, device=dev)
+        rows = torch.empty(max(saved, 1) * cols, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        if saved:
+            _device_copy(slots.data_ptr(), d_ids.value, 4 * saved)
+            _device_copy(w.data_ptr(), d_w.value, 4 * saved)
+            _device_copy(rows.data_ptr(), d_rows.value, 4 * saved * cols)
+        slots, w = slots[:saved], w[:saved]
+        slot64 = slots.to(torch.int64) & 0xFFFFFFFF
+        src = slot64 // N
+        det = rows[:saved * cols].view(saved, cols)[:, 0].to(torch.int64) - 1
+        pair = src * D + det
+        I = torch.zeros(S * D, dtype=torch.float64, device=dev).index_add_(0, pair, w.double())
+        w2 = torch.zeros(S * D, dtype=torch.float64, device=dev).index_add_(0, pair, w.double() ** 2)
+        self._dev = {"ids": (slot64 - src * N).to(torch.int32), "src": src.to(torch.int32), "pair": pair,
+                     "w": w, "I": I, "w2": w2, "table": torch.from_numpy(table).to(dev)}
+        self.nrecords = int(saved)
+        self.I = I.view(S, D).cpu().numpy()
+        self._records = None
         with np.errstate(divide="ignore"):
-            return np.log(I)
+            return np.log(self.I)
+
+    @property
+    def records(self):
+        """Per source (photon ids int32, detector int64, final weight float32) host arrays of the
+        last forward, copied from the GPU on first use."""
+        if self._records is None:
+            d = self._dev
+            src = d["src"].cpu().numpy().astype(np.int64)
+            ids, det, w = d["ids"].cpu().numpy(), (d["pair"].cpu().numpy() - src * self.ndet), d["w"].cpu().numpy()
+            order = np.argsort(src, kind="stable")
+            src, ids, det, w = src[order], ids[order], det[order], w[order]
+            bounds = np.searchsorted(src, np.arange(len(self.sources) + 1))
+            self._records = [(ids[a:b], det[a:b], w[a:b]) for a, b in zip(bounds[:-1], bounds[1:])]
+        return self._records
+
+    def noise(self) -> np.ndarray:
+        """Relative standard deviation of each I_sd, sqrt(sum w^2) / sum w (NaN where nothing was
+        detected), shape (sources, detectors)."""
+        d = self._dev
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return (d["w2"].sqrt() / d["I"]).view(len(self.sources), self.ndet).cpu().numpy()
 
     # -- backward -------------------------------------------------------------------------------
     def _replay(self, s, coef, slices, gmua, gmus, final_w=None):
@@ -131,44 +186,47 @@ class MultiSourceModel:
                                     C.c_void_p(final_w.data_ptr()) if final_w is not None else None, C.byref(ms)))
         return ms.value
 
-    def _replay_all(self, coefs, slices, gmua, gmus):
-        """Replay the records of every source in one launch (umcx_session_replay_sources)."""
+    def _replay_all(self, coef, slices, gmua, gmus):
+        """Replay the records of every source in one launch (umcx_session_replay_sources);
+        coef (float32) and slices (int32) are device tensors, one entry per record."""
         torch = _torch()
-        dev = f"cuda:{self.device}"
-        ids = np.concatenate([r[0] for r in self.records]) if self.records else np.zeros(0, np.int32)
-        n = len(ids)
+        d = self._dev
+        n = self.nrecords
         if n == 0:
             return 0.0
-        src = np.concatenate([np.full(len(r[0]), s, np.int32) for s, r in enumerate(self.records)])
-        table = np.array([list(p) + list(d) for p, d in self.sources], dtype=np.float32).reshape(-1)
-        t_ids = torch.from_numpy(ids.astype(np.int32)).to(dev)
-        t_sl = torch.from_numpy(np.ascontiguousarray(np.concatenate(slices), dtype=np.int32)).to(dev)
-        t_cf = torch.from_numpy(np.ascontiguousarray(np.concatenate(coefs), dtype=np.float32)).to(dev)
-        t_src = torch.from_numpy(src).to(dev)
-        t_tab = torch.from_numpy(table).to(dev)
+        coef = coef.to(torch.float32).contiguous()
+        slices = slices.to(torch.int32).contiguous()
         torch.cuda.synchronize()
         ms = C.c_double(0.0)
         check(lib().umcx_session_replay_sources(
-            self.sim._h, C.c_void_p(t_ids.data_ptr()), C.c_void_p(t_sl.data_ptr()), C.c_void_p(t_cf.data_ptr()),
-            C.c_void_p(t_src.data_ptr()), C.c_void_p(t_tab.data_ptr()), n, C.c_void_p(gmua.data_ptr()),
+            self.sim._h, C.c_void_p(d["ids"].data_ptr()), C.c_void_p(slices.data_ptr()), C.c_void_p(coef.data_ptr()),
+            C.c_void_p(d["src"].data_ptr()), C.c_void_p(d["table"].data_ptr()), n, C.c_void_p(gmua.data_ptr()),
             C.c_void_p(gmus.data_ptr()), self.nvox, None, C.byref(ms)))
         return ms.value
+
+    def _coef(self, weights=None):
+        """w / I_sd per record (times weights[s, d] when given), 0 where I_sd = 0; on the GPU."""
+        torch = _torch()
+        d = self._dev
+        Ipair = d["I"][d["pair"]]
+        num = d["w"].double()
+        if weights is not None:
+            num = num * weights.reshape(-1)[d["pair"]]
+        return torch.where(Ipair > 0, num / torch.where(Ipair > 0, Ipair, torch.ones_like(Ipair)),
+                           torch.zeros_like(Ipair))
 
     def vjp(self, grad_logI):
         """sum_sd grad[s,d] * d log I_sd / d(mua, mus): two (nx, ny, nz) float32 cuda tensors."""
         torch = _torch()
-        g = np.asarray(grad_logI.detach().cpu().numpy() if hasattr(grad_logI, "detach") else grad_logI,
-                       dtype=np.float64)
         dev = f"cuda:{self.device}"
+        if hasattr(grad_logI, "detach"):
+            g = grad_logI.detach().to(device=dev, dtype=torch.float64)
+        else:
+            g = torch.as_tensor(np.asarray(grad_logI, dtype=np.float64), device=dev)
         gmua = torch.zeros(self.nvox, dtype=torch.float32, device=dev)
         gmus = torch.zeros(self.nvox, dtype=torch.float32, device=dev)
-        coefs, slices = [], []
-        for s in range(len(self.sources)):
-            ids, det, w = self.records[s]
-            Isd = self.I[s, det]
-            coefs.append(np.where(Isd > 0, w * g[s, det] / np.where(Isd > 0, Isd, 1.0), 0.0))
-            slices.append(np.zeros(len(ids), np.int32))
-        self.replay_ms = self._replay_all(coefs, slices, gmua, gmus)
+        coef = self._coef(g)
+        self.replay_ms = self._replay_all(coef, torch.zeros_like(self._dev["src"]), gmua, gmus)
         return gmua.view(self.shape), gmus.view(self.shape)
 
     def jacobian(self):
@@ -177,13 +235,7 @@ class MultiSourceModel:
         dev = f"cuda:{self.device}"
         S, D = len(self.sources), self.ndet
         J = torch.zeros((2, S * D, self.nvox), dtype=torch.float32, device=dev)
-        coefs, slices = [], []
-        for s in range(S):
-            ids, det, w = self.records[s]
-            Isd = self.I[s, det]
-            coefs.append(np.where(Isd > 0, w / np.where(Isd > 0, Isd, 1.0), 0.0))
-            slices.append((s * D + det).astype(np.int32))
-        self.replay_ms = self._replay_all(coefs, slices, J[0], J[1])
+        self.replay_ms = self._replay_all(self._coef(), self._dev["pair"], J[0], J[1])
         return J.permute(1, 2, 0)
 
     def replay_weights(self, s: int):
