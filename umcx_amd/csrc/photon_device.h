@@ -70,6 +70,7 @@ struct Photon {
     uint32_t med;
     int tvox;                  // offset of lastvox inside the LDS tally tile, -1 outside it
     float cmua, cmus;          // absorption and scattering coefficients of lastvox
+    float cn;                  // refractive index of lastvox
 };
 
 // Label sources. raw() issues the load (its value is waited for where decode() uses it), decode()

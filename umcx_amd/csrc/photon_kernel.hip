@@ -48,6 +48,9 @@ constexpr int kRegPaths = 4;                       // partial paths kept in regi
 constexpr uint64_t kShortRunPhotonsPerLane = 12;
 constexpr int kHbmLabelWgPerCu = 4;                // workgroups per CU when labels are read from HBM
 constexpr int kDepCache = 4;                       // global deposits cached per lane
+// lanes whose photons ended wait until at least this many of their wave's lanes need a new photon
+// (or no lane of the wave has one left) before the wave takes new photon slots
+constexpr int kRefillMin = 2;
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
 
@@ -138,14 +141,15 @@ __device__ __forceinline__ int tile_local(const KernelArgs& a, int ix, int iy, i
     return (dx < a.tsx && dy < a.tsy && dz < a.tsz) ? (int)((dx * a.tsy + dy) * a.tsz + dz) : -1;
 }
 
-// coefficients of the photon's current voxel: the label's medium, or the per-voxel override
-__device__ __forceinline__ void load_props(Photon& p, const KernelArgs& a, const float4* smedia) {
+// coefficients of the photon's current voxel: the label's medium, or the per-voxel override of
+// mua and mus (the refractive index stays the label's)
+__device__ __forceinline__ void set_props(Photon& p, const KernelArgs& a, const float4 m) {
+    p.cn = m.w;
     if (a.vprops) {
         const float2 v = a.vprops[p.lastvox];
         p.cmua = v.x;
         p.cmus = v.y;
     } else {
-        const float4 m = smedia[p.med & kMed];
         p.cmua = m.x;
         p.cmus = m.y;
     }
@@ -210,7 +214,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         // ---- take a new photon when this lane's has ended (one atomic per wave) ----
         // The workgroup's static slots come through an LDS counter, so a refill does not wait on
         // a device atomic (whose wait would also cover every fluence atomic still in flight).
-        if (!alive) {
+        // A refill is a long stretch of scalar and LDS round trips for the few lanes that take
+        // part, so a lane whose photon ended waits until kRefillMin lanes of its wave need one
+        // (or none of them has a photon left); no photon's trajectory depends on when it starts.
+        const unsigned long long dead = __ballot(!alive);
+        if (!alive && (__popcll(dead) >= kRefillMin || dead == __ballot(1))) {
             const ArgsRef ca = reread_args();
             Source src{ld4(&ca->srcpos), ld4(&ca->srcdir), ld4(&ca->srcp1), ld4(&ca->srcp2), ca->srctype};
             const unsigned long long need = __ballot(1);
@@ -275,10 +283,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
             }
             p.tvox = tile_local(a, p.ix, p.iy, p.iz);
             p.med = labs.decode(labs.raw(p.lastvox));                                                // :269
-            load_props(p, a, smedia);
+            set_props(p, a, smedia[p.med & kMed]);
             p.lx = rng_scatlen(r, tabs);                                                             // :270
             alive = true;
         }
+        if (!alive) continue;          // waiting for its wave's next refill
 
         bool ended = false;
         // vote > 0: a scatter round runs only when enough lanes wait to scatter; in between, the
@@ -290,9 +299,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         }
         // ---- one step of sprint (:282-311) ----
         if (p.lx > 0.f && (a.vote == 0 || !scatter_round)) {
-            const float4 prop = smedia[p.med & kMed];
             uint32_t nlab;
-            int nv = step(p, prop.w, grid, tabs, labs, nlab);
+            int nv = step(p, p.cn, grid, tabs, labs, nlab);
             if (nv != p.lastvox) {
                 const uint32_t m = p.med & kMed;
                 if (REPLAY) {
@@ -340,8 +348,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                 } else {
                     uint32_t nm = (nv >= 0) ? labs.decode(nlab) : 0u;
                     const float4 nprop = smedia[nm & kMed];
-                    if (REFLECT && prop.w != nprop.w) {
-                        if (boundary(p, prop.w, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;
+                    if (REFLECT && p.cn != nprop.w) {
+                        if (boundary(p, p.cn, nprop.w, r, nv, nm) && (nv < 0 || (nm & kMed) == 0)) ended = true;
                     } else if (nv < 0 || ((nm & kMed) == 0 && m != 0)) {
                         ended = true;
                     }
@@ -350,7 +358,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                         if (moved) p.tvox = tile_local(a, p.ix, p.iy, p.iz);
                         p.lastvox = nv;
                         p.med = nm;
-                        if (moved) load_props(p, a, smedia);
+                        if (moved) set_props(p, a, nprop);
                     }
                 }
             }
