@@ -52,6 +52,20 @@ def _device_copy(dst: int, src: int, nbytes: int) -> None:
         raise RuntimeError(f"hipMemcpy device to device failed ({rc})")
 
 
+def _pair_sums(pair, w, n: int):
+    """sum of w and of w^2 per (source, detector) pair index in [0, n), on the GPU: sorted by pair,
+    prefix sums, differences at the segment bounds (float atomics into a few hundred bins would
+    serialise on the same addresses)."""
+    torch = _torch()
+    order = torch.argsort(pair)
+    ps, ws = pair[order], w[order]
+    zero = torch.zeros(1, dtype=w.dtype, device=w.device)
+    c1 = torch.cat([zero, torch.cumsum(ws, 0)])
+    c2 = torch.cat([zero, torch.cumsum(ws * ws, 0)])
+    bounds = torch.searchsorted(ps, torch.arange(n + 1, dtype=ps.dtype, device=ps.device))
+    return c1[bounds[1:]] - c1[bounds[:-1]], c2[bounds[1:]] - c2[bounds[:-1]]
+
+
 class MultiSourceModel:
     """One volume, several pencil sources and the configuration's detectors.
 
@@ -133,8 +147,7 @@ class MultiSourceModel:
         src = slot64 // N
         det = rows[:saved * cols].view(saved, cols)[:, 0].to(torch.int64) - 1
         pair = src * D + det
-        I = torch.zeros(S * D, dtype=torch.float64, device=dev).index_add_(0, pair, w.double())
-        w2 = torch.zeros(S * D, dtype=torch.float64, device=dev).index_add_(0, pair, w.double() ** 2)
+        I, w2 = _pair_sums(pair, w.double(), S * D)
         self._dev = {"ids": (slot64 - src * N).to(torch.int32), "src": src.to(torch.int32), "pair": pair,
                      "w": w, "I": I, "w2": w2, "table": torch.from_numpy(table).to(dev)}
         self.nrecords = int(saved)
