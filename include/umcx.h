@@ -147,9 +147,9 @@ void umcx_session_free(umcx_session* s);
 
 /* ---- batched simulations (no counterpart in the reference) ------------------------------- */
 
-/* Many independent simulations (each a complete umcx JSON config; fluence output only) run in
- * one persistent launch replayed from a hipGraph. Per-job results equal separate runs up to float
- * summation order. */
+/* Many independent simulations (each a complete umcx JSON config) run in one persistent launch
+ * replayed from a hipGraph. Per-job results (fluence, energies, detected photons) equal separate
+ * runs up to float summation order; detected rows equal them as a multiset. */
 typedef struct umcx_batch umcx_batch;
 int umcx_batch_create(const char* const* jsons, int32_t njobs, const umcx_launch* launch, umcx_batch** out);
 /* Zero the tallies and run every job's photons; kernel_ms (may be NULL) gets the graph's time. */
@@ -157,6 +157,12 @@ int umcx_batch_run(umcx_batch* b, double* kernel_ms);
 int umcx_batch_info(const umcx_batch* b, int32_t job, umcx_info* info);
 int umcx_batch_fluence(const umcx_batch* b, int32_t job, float* out, uint64_t n);
 int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]);
+/* Detected photons of one job (src/umcx.cpp:155-184): all detections and the rows kept (at most
+ * the job's MaxDetPhoton), then the rows [saved x detp_columns] in the layout of :164-176 with
+ * the photon index and final weight of each (any pointer may be NULL), as umcx_session_detected /
+ * umcx_session_detp give them for a separate run. */
+int umcx_batch_detected(const umcx_batch* b, int32_t job, uint32_t* detected, uint32_t* saved);
+int umcx_batch_detp(const umcx_batch* b, int32_t job, float* rows, uint32_t* ids, float* weights);
 /* One launch shape the autotuner timed at creation: workgroup size, persistent workgroups, LDS
  * tally cells per workgroup (0: global atomics only), the trial's kernel time over a 1/32 sample
  * of every job (0 when an explicit shape leaves a single candidate, which is not timed), the share

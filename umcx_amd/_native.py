@@ -87,6 +87,8 @@ SIGNATURES = {
     "umcx_batch_info": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(UmcxInfo)]),
     "umcx_batch_fluence": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64]),
     "umcx_batch_energies": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "umcx_batch_detected": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "umcx_batch_detp": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "umcx_batch_trials": (C.c_int, [C.c_void_p, C.POINTER(UmcxBatchTrial), C.c_int32, C.POINTER(C.c_int32)]),
     "umcx_batch_tally_box": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "umcx_batch_free": (None, [C.c_void_p]),

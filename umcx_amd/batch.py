@@ -107,6 +107,24 @@ class BatchSimulation:
         check(lib().umcx_batch_fluence(self._h, j, out.ctypes.data, out.size))
         return out.reshape(nt, nx, ny, nz)
 
+    def detected(self, j: int) -> tuple[int, int]:
+        """(all detections, rows kept) of job j."""
+        d, sv = C.c_uint32(), C.c_uint32()
+        check(lib().umcx_batch_detected(self._h, j, C.byref(d), C.byref(sv)))
+        return d.value, sv.value
+
+    def detp(self, j: int):
+        """Detected-photon rows of job j [saved, detp_columns], their photon indices and final
+        weights, as Simulation.detp gives them for a separate run."""
+        _, saved = self.detected(j)
+        cols = self.infos[j]["detp_columns"]
+        rows = np.empty((saved, cols), dtype=np.float32)
+        ids = np.empty(saved, dtype=np.uint32)
+        w = np.empty(saved, dtype=np.float32)
+        check(lib().umcx_batch_detp(self._h, j, rows.ctypes.data if rows.size else None,
+                                    ids.ctypes.data if saved else None, w.ctypes.data if saved else None))
+        return rows, ids, w
+
     def energies(self, j: int):
         e = (C.c_double * 3)()
         check(lib().umcx_batch_energies(self._h, j, e))

@@ -47,6 +47,11 @@ public:
     uint64_t fluence_len(int j) const;
     void fetch_fluence(int j, float* out) const;      // raw tallies [t][x][y][z]
     void energies(int j, double out[3]) const;        // escaped, launched, deposited
+    // detected photons of job j (:155-184): all detections and rows kept (<= its MaxDetPhoton),
+    // rows [saved x detp_columns] plus the photon index and final weight of each (any may be null)
+    uint32_t detected(int j) const;
+    uint32_t saved(int j) const;
+    void fetch_detp(int j, float* rows, uint32_t* ids, float* weights) const;
     int block() const { return chosen_.block; }
     int blocks() const { return chosen_.blocks; }
     int tile_cells() const { return chosen_.tile_cells; }
@@ -78,6 +83,24 @@ private:
     float* d_fluence_ = nullptr;
     double* d_energy_ = nullptr;
     int first_cap_ = 0;
+    // detected photons: per job, its detectors' offset and count in the detector table and its
+    // row slots (the first slot's index in the id and weight buffers, its float offset in the
+    // row buffer)
+    struct JobRows {
+        uint32_t det_off = 0, detnum = 0, maxdet = 0, detlen = 0, ppathlen = 0;
+        uint64_t row_off = 0, detp_off = 0;
+    };
+    std::vector<JobRows> rows_;
+    uint64_t rows_total_ = 0, detp_floats_ = 0;
+    uint32_t pp_max_ = 0;
+    bool any_det_ = false;
+    uint32_t* d_ndet_ = nullptr;
+    float* d_detpos_ = nullptr;
+    float* d_detp_ = nullptr;
+    uint32_t* d_detid_ = nullptr;
+    float* d_detw_ = nullptr;
+    size_t lds_fixed(int block) const;
+    void set_rows(void* args) const;
 
     // upload a job table (nphoton scaled down to `sample` photons when nonzero, tiles from boxes)
     // and the workgroups' initial jobs, then launch once on the session stream

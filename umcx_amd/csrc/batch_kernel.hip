@@ -46,7 +46,15 @@ struct __align__(16) JobDesc {
     int32_t tx0, ty0, tz0, tg0;        // LDS tally box origin (voxel, gate)
     uint32_t tsx, tsy, tsz, tsg;       // and extent; tsx == 0: no box
     float cost;                        // estimated work per photon (scheduling weight, > 0)
-    uint32_t pad[3];
+    // detected photons (:155-184): detectors at det_off of the detector table; maxdet row slots
+    // starting at slot row_off of the id and weight buffers and at float detp_off of the row
+    // buffer, detlen floats each, columns from savedetflag
+    uint32_t detnum, det_off, maxdet;
+    uint32_t detlen;
+    uint64_t row_off, detp_off;
+    uint32_t ppathlen;
+    int32_t savedetflag;
+    uint32_t pad[2];
 };
 
 struct BatchArgs {
@@ -61,6 +69,14 @@ struct BatchArgs {
     unsigned long long* jcur;   // photon cursor per job
     uint32_t* active;           // workgroups on each job
     int32_t count_mode;         // 1: tally the number of deposits (pilot)
+    // detected photons (SAVEDET kernels): every job's detectors, the row buffers and one
+    // detection counter per job
+    const float4* detpos;
+    float* detp;                // rows; job j's start at its detp_off
+    uint32_t* detid;            // photon index of each row (within its job)
+    float* detw;                // final weight of each row
+    uint32_t* ndet;             // detections per job, including ones beyond its maxdet
+    uint32_t pp_max;            // partial-path columns per lane in LDS (the largest ppathlen)
 };
 
 __device__ __forceinline__ int box_local(int ix, int iy, int iz, int x0, int y0, int z0, uint32_t sx, uint32_t sy,
@@ -97,14 +113,18 @@ __device__ int pick_job(const BatchArgs& b) {
     return bj;
 }
 
+// SAVEDET: jobs may save detected photons (their partial paths per medium live per lane in LDS,
+// after the tally tile)
+template <bool SAVEDET>
 __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
-    // LDS: libm tables | the current job's media | tally tile
+    // LDS: libm tables | the current job's media | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
     __shared__ int s_job;
     libm::LogfEntry* slog = (libm::LogfEntry*)lds4;
     uint64_t* sexp = (uint64_t*)(lds4 + 16);
     float4* smedia = lds4 + 32;
     float* stile = (float*)(lds4 + 32 + kMaxBatchMedia);
+    float* sppath = stile + b.tile_cap;
     const int tid = threadIdx.x, bd = blockDim.x;
     if (tid < 16) slog[tid] = libm::kLogfTab[tid];
     if (tid < 32) sexp[tid] = libm::kExp2Tab[tid];
@@ -131,12 +151,14 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
         const uint32_t tile_n = (b.tile_cap && tcells * tsg <= b.tile_cap) ? tcells * tsg : 0u;
         const uint32_t tsx = tile_n ? J->tsx : 0u;
         const uint32_t nmed = J->mediumnum;
+        const uint32_t ppathlen = SAVEDET ? J->ppathlen : 0u;
         for (uint32_t i = tid; i < nmed; i += bd) smedia[i] = b.media[J->media_off + i];
         for (uint32_t i = tid; i < tile_n; i += bd) stile[i] = 0.f;
         __syncthreads();
 
         Photon p;
         Rng r;
+        uint64_t pid = 0;
         bool alive = false, dry = false;
         float w0 = 0.f;
         double e_esc = 0.0, e_tot = 0.0, e_dep = 0.0, ph_dep = 0.0;
@@ -158,6 +180,9 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
                     const Source src{J->srcpos, J->srcdir, J->srcp1, J->srcp2, J->srctype};
                     w0 = launch(p, r, src);
                     ph_dep = 0.0;
+                    pid = k;
+                    if (SAVEDET)
+                        for (uint32_t m = 0; m < ppathlen; m++) sppath[m * bd + tid] = 0.f;
                     p.lastvox = vox_index(g, p.ix, p.iy, p.iz);
                     bool entered = p.lastvox >= 0 || skip_to_box(p, g);
                     if (entered && p.lastvox < 0) entered = false;
@@ -183,6 +208,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
                 int nv = step(p, prop.w, g, tabs, GlobalLabels<uint8_t>{b.labels + lab_off}, nlab);
                 if (nv != p.lastvox) {
                     const uint32_t mcur = p.med & kMed;
+                    if (SAVEDET && ppathlen && mcur > 0) sppath[(mcur - 1) * bd + tid] += p.lz;   // :286-288
                     if (savevol) {
                         int tshift = (int)fminf((float)(maxgate - 1), (float)(int)floorf((p.ly - tstart) * rtstep));
                         float dw = p.lw - p.pw;
@@ -226,6 +252,27 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
             if (!ended) {
                 if (!(p.lx > 0.f)) scatter(p, r, smedia[p.med & kMed], tabs);
                 continue;
+            }
+            // detection (:276-278, :429-435, :164-176)
+            if (SAVEDET && J->detnum && p.ly <= tend && (p.med & kDet)) {
+                for (uint32_t d = 0; d < J->detnum; d++) {
+                    const float4 dp = b.detpos[J->det_off + d];
+                    const float dx = dp.x - p.px, dy = dp.y - p.py, dz = dp.z - p.pz;
+                    if (dx * dx + dy * dy + dz * dz < dp.w * dp.w) {
+                        const uint32_t slot = atomicAdd(b.ndet + job, 1u);
+                        if (slot < J->maxdet) {
+                            const uint64_t rowi = J->row_off + slot;
+                            float* row = b.detp + J->detp_off + (uint64_t)slot * J->detlen;
+                            uint32_t c = 0;
+                            if (J->savedetflag & kDetID) row[c++] = (float)(d + 1);
+                            for (uint32_t m = 0; m < ppathlen; m++) row[c++] = sppath[m * bd + tid];
+                            if (J->savedetflag & kExitPos) { row[c++] = p.px; row[c++] = p.py; row[c++] = p.pz; }
+                            if (J->savedetflag & kExitDir) { row[c++] = p.vx; row[c++] = p.vy; row[c++] = p.vz; }
+                            b.detid[rowi] = (uint32_t)pid;
+                            b.detw[rowi] = p.pw;
+                        }
+                    }
+                }
             }
             if (p.med & kMed) {                                                                      // :765-770
                 e_tot += (double)w0;
@@ -274,14 +321,15 @@ __global__ void fill_zero_f32(float* p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = 0.f;
 }
-// zero the per-job energies, cursors and workgroup counts
-__global__ void zero_jobs(double* energy, unsigned long long* jcur, uint32_t* active, uint32_t njobs) {
+// zero the per-job energies, cursors, workgroup counts and detection counters
+__global__ void zero_jobs(double* energy, unsigned long long* jcur, uint32_t* active, uint32_t* ndet, uint32_t njobs) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < njobs; i += gridDim.x * blockDim.x) {
         energy[3 * i] = 0.0;
         energy[3 * i + 1] = 0.0;
         energy[3 * i + 2] = 0.0;
         jcur[i] = 0ull;
         active[i] = 0u;
+        ndet[i] = 0u;
     }
 }
 
@@ -323,13 +371,12 @@ BatchSession::BatchSession(const std::vector<Problem>& jobs, const LaunchConfig&
     uint64_t lab_total = 0;
     for (const Problem& p : jobs) lab_total += (uint64_t)p.nx * p.ny * p.nz;
     std::vector<uint8_t> labels(lab_total);
+    std::vector<float> detpos;
     uint64_t lab_off = 0;
     flu_off_.resize(jobs.size());
     for (size_t j = 0; j < jobs.size(); j++) {
         const Problem& p = jobs[j];
         if (p.mediumnum > (int)dev::kMaxBatchMedia) throw std::runtime_error("batched jobs are limited to 128 media each");
-        if (p.issavedet && p.detnum > 0)
-            throw std::runtime_error("batched jobs tally fluence only; set DoPartialPath 0 or drop the detectors");
         if (p.nphoton > 0xFFFFFFFFull) throw std::runtime_error("batched jobs are limited to 2^32 photons each");
         const uint64_t nxyz = (uint64_t)p.nx * p.ny * p.nz;
         for (uint64_t i = 0; i < nxyz; i++) {
@@ -337,6 +384,23 @@ BatchSession::BatchSession(const std::vector<Problem>& jobs, const LaunchConfig&
             labels[lab_off + i] = (uint8_t)((v & kMedMask) | ((v & kDetMask) ? 0x80u : 0u));
         }
         media.insert(media.end(), p.media.begin(), p.media.end());
+        // detected photons: this job's detectors and row slots
+        JobRows jr;
+        if (p.issavedet && p.detnum > 0) {
+            jr.det_off = (uint32_t)(detpos.size() / 4);
+            jr.detnum = (uint32_t)p.detnum;
+            detpos.insert(detpos.end(), p.detpos.begin(), p.detpos.end());
+            jr.row_off = rows_total_;
+            jr.detp_off = detp_floats_;
+            jr.maxdet = p.maxdetphotons;
+            jr.detlen = (uint32_t)p.detp_columns();
+            jr.ppathlen = (uint32_t)p.ppath_columns();
+            detp_floats_ += (uint64_t)jr.maxdet * jr.detlen;
+            rows_total_ += jr.maxdet;
+            pp_max_ = std::max(pp_max_, jr.ppathlen);
+            any_det_ = true;
+        }
+        rows_.push_back(jr);
         lab_off += nxyz;
         flu_off_[j] = flu_total_;
         flu_total_ += p.issavevol ? nxyz * (uint64_t)p.maxgate : 0;
@@ -350,6 +414,12 @@ BatchSession::BatchSession(const std::vector<Problem>& jobs, const LaunchConfig&
     d_energy_ = dalloc_b<double>(3 * jobs.size());
     d_jcur_ = dalloc_b<unsigned long long>(jobs.size());
     d_active_ = dalloc_b<uint32_t>(jobs.size());
+    d_ndet_ = dalloc_b<uint32_t>(jobs.size());
+    d_detpos_ = dalloc_b<float>(std::max<size_t>(detpos.size(), 4));
+    if (!detpos.empty()) HIP_OK(hipMemcpy(d_detpos_, detpos.data(), detpos.size() * 4, hipMemcpyHostToDevice));
+    d_detp_ = dalloc_b<float>(detp_floats_);
+    d_detid_ = dalloc_b<uint32_t>(rows_total_);
+    d_detw_ = dalloc_b<float>(rows_total_);
     boxes_.assign(jobs.size(), TallyBox{0, 0, 0, 0, 0, 0, 0, 0});
     cost_.assign(jobs.size(), 1.0);
 
@@ -377,12 +447,15 @@ BatchSession::BatchSession(const std::vector<Problem>& jobs, const LaunchConfig&
     ba.jcur = d_jcur_;
     ba.active = d_active_;
     ba.count_mode = 0;
-    const size_t lds = kLdsFixed + (size_t)chosen_.tile_cells * 4;
+    set_rows(&ba);
+    const size_t lds = lds_fixed(chosen_.block) + (size_t)chosen_.tile_cells * 4;
     hipGraph_t graph;
     HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     if (flu_total_) hipLaunchKernelGGL(dev::fill_zero_f32, dim3(1024), dim3(256), 0, s, d_fluence_, flu_total_);
-    hipLaunchKernelGGL(dev::zero_jobs, dim3(4), dim3(256), 0, s, d_energy_, d_jcur_, d_active_, (uint32_t)jobs.size());
-    hipLaunchKernelGGL(dev::batch_kernel, dim3(chosen_.blocks), dim3(chosen_.block), lds, s, ba);
+    hipLaunchKernelGGL(dev::zero_jobs, dim3(4), dim3(256), 0, s, d_energy_, d_jcur_, d_active_, d_ndet_,
+                       (uint32_t)jobs.size());
+    if (any_det_) hipLaunchKernelGGL(dev::batch_kernel<true>, dim3(chosen_.blocks), dim3(chosen_.block), lds, s, ba);
+    else hipLaunchKernelGGL(dev::batch_kernel<false>, dim3(chosen_.blocks), dim3(chosen_.block), lds, s, ba);
     HIP_OK(hipStreamEndCapture(s, &graph));
     graph_ = graph;
     hipGraphExec_t exec;
@@ -396,7 +469,8 @@ BatchSession::~BatchSession() {
     if (graph_exec_) (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
     if (graph_) (void)hipGraphDestroy((hipGraph_t)graph_);
     for (void* ptr : {d_labels_, d_media_, d_jobs_, (void*)d_first_, (void*)d_jcur_, (void*)d_active_,
-                      (void*)d_fluence_, (void*)d_energy_})
+                      (void*)d_fluence_, (void*)d_energy_, (void*)d_ndet_, (void*)d_detpos_, (void*)d_detp_,
+                      (void*)d_detid_, (void*)d_detw_})
         if (ptr) (void)hipFree(ptr);
     if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
     if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
@@ -437,6 +511,9 @@ void BatchSession::upload_jobs(const std::vector<TallyBox>* boxes, uint64_t div,
             d.tsx = bx[4]; d.tsy = bx[5]; d.tsz = bx[6]; d.tsg = bx[7];
         }
         d.cost = (float)(cost_[j] + kCostBase);
+        const JobRows& jr = rows_[j];
+        d.detnum = jr.detnum; d.det_off = jr.det_off; d.maxdet = jr.maxdet; d.row_off = jr.row_off;
+        d.detp_off = jr.detp_off; d.detlen = jr.detlen; d.ppathlen = jr.ppathlen; d.savedetflag = p.savedetflag;
         work[j] = (double)n * d.cost;
         lab_off += d.nxyz;
         med_off += (uint32_t)p.mediumnum;
@@ -482,7 +559,8 @@ void BatchSession::upload_first(int blocks, const std::vector<double>& work) {
 void BatchSession::zero_state(bool fluence) {
     hipStream_t s = (hipStream_t)stream_;
     if (fluence && flu_total_) hipLaunchKernelGGL(dev::fill_zero_f32, dim3(1024), dim3(256), 0, s, d_fluence_, flu_total_);
-    hipLaunchKernelGGL(dev::zero_jobs, dim3(4), dim3(256), 0, s, d_energy_, d_jcur_, d_active_, (uint32_t)probs_.size());
+    hipLaunchKernelGGL(dev::zero_jobs, dim3(4), dim3(256), 0, s, d_energy_, d_jcur_, d_active_, d_ndet_,
+                       (uint32_t)probs_.size());
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s));
 }
@@ -506,9 +584,13 @@ double BatchSession::launch_once(int block, int blocks, int tile_cells, const st
     ba.jcur = d_jcur_;
     ba.active = d_active_;
     ba.count_mode = count_mode;
-    const size_t lds = kLdsFixed + (size_t)tile_cells * 4;
+    set_rows(&ba);
+    // the counting pilot needs no detected photons; trials time the kernel the runs use
+    const bool det = any_det_ && !count_mode;
+    const size_t lds = (det ? lds_fixed(block) : kLdsFixed) + (size_t)tile_cells * 4;
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    hipLaunchKernelGGL(dev::batch_kernel, dim3(blocks), dim3(block), lds, s, ba);
+    if (det) hipLaunchKernelGGL(dev::batch_kernel<true>, dim3(blocks), dim3(block), lds, s, ba);
+    else hipLaunchKernelGGL(dev::batch_kernel<false>, dim3(blocks), dim3(block), lds, s, ba);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -527,14 +609,17 @@ void BatchSession::tune() {
     HIP_OK(hipGetDeviceProperties(&prop, lc_.device));
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 160 * 1024;
     const size_t lds_wg = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 64 * 1024;
+    const void* kfn = any_det_ ? (const void*)dev::batch_kernel<true> : (const void*)dev::batch_kernel<false>;
     auto occupancy = [&](int block, size_t lds) {
         int per_cu = 0;
-        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)dev::batch_kernel, block, lds));
+        HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, block, lds));
         return std::max(per_cu, 0);
     };
 
     // 1. counting pilot (no tile): deposits per photon and the deposit map of every job
-    const int pblock = 256, pblocks = cus_ * std::max(1, occupancy(256, kLdsFixed));
+    int pocc = 0;
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pocc, (const void*)dev::batch_kernel<false>, 256, kLdsFixed));
+    const int pblock = 256, pblocks = cus_ * std::max(1, pocc);
     launch_once(pblock, std::min(pblocks, first_cap_), 0, nullptr, 1, kPilotPhotons, kPilotPhotons, 1);
     std::unique_ptr<float[]> counts(new float[std::max<uint64_t>(flu_total_, 1)]);   // filled by the copy
     if (flu_total_) HIP_OK(hipMemcpy(counts.get(), d_fluence_, flu_total_ * 4, hipMemcpyDeviceToHost));
@@ -555,7 +640,9 @@ void BatchSession::tune() {
     if (lc_.block > 0) blocks_list = {std::max(64, std::min(1024, (lc_.block / 64) * 64))};
     std::vector<BatchTrial> cands;
     for (int block : blocks_list) {
-        const int occ = occupancy(block, kLdsFixed);
+        const size_t fixed = lds_fixed(block);
+        if (fixed > lds_wg) continue;
+        const int occ = occupancy(block, fixed);
         if (occ < 1) continue;
         for (int per_cu = occ; per_cu >= std::max(1, occ - 1); per_cu--) {
             BatchTrial t;
@@ -563,11 +650,11 @@ void BatchSession::tune() {
             t.blocks = lc_.blocks > 0 ? lc_.blocks : cus_ * per_cu;
             const int wg_cu = lc_.blocks > 0 ? (lc_.blocks + cus_ - 1) / cus_ : per_cu;
             const size_t share = std::min(lds_wg, lds_cu / (size_t)std::max(1, wg_cu));
-            int cells = share > kLdsFixed ? (int)std::min<size_t>((share - kLdsFixed) / 4, kMaxTileCells) : 0;
+            int cells = share > fixed ? (int)std::min<size_t>((share - fixed) / 4, kMaxTileCells) : 0;
             cells = (cells / 256) * 256;
             if (lc_.tally_mode == 0) cells = 0;
             // largest tile that keeps the wanted workgroups per CU resident
-            while (cells > 0 && occupancy(block, kLdsFixed + (size_t)cells * 4) < std::min(per_cu, occ)) cells -= 256;
+            while (cells > 0 && occupancy(block, fixed + (size_t)cells * 4) < std::min(per_cu, occ)) cells -= 256;
             t.tile_cells = cells;
             bool dup = false;
             for (const BatchTrial& c : cands) dup |= c.block == t.block && c.blocks == t.blocks && c.tile_cells == t.tile_cells;
@@ -651,6 +738,42 @@ void BatchSession::fetch_fluence(int j, float* out) const {
     HIP_OK(hipSetDevice(lc_.device));
     uint64_t n = fluence_len(j);
     if (n) HIP_OK(hipMemcpy(out, d_fluence_ + flu_off_.at(j), n * 4, hipMemcpyDeviceToHost));
+}
+
+// Dynamic LDS beside the tally tile: libm tables and media, plus the per-lane partial paths when
+// any job saves detected photons.
+size_t BatchSession::lds_fixed(int block) const {
+    return kLdsFixed + (any_det_ ? (size_t)pp_max_ * (size_t)block * 4 : 0);
+}
+
+void BatchSession::set_rows(void* args) const {
+    dev::BatchArgs& ba = *(dev::BatchArgs*)args;
+    ba.detpos = (const float4*)d_detpos_;
+    ba.detp = d_detp_;
+    ba.detid = d_detid_;
+    ba.detw = d_detw_;
+    ba.ndet = d_ndet_;
+    ba.pp_max = pp_max_;
+}
+
+uint32_t BatchSession::detected(int j) const {
+    HIP_OK(hipSetDevice(lc_.device));
+    if (j < 0 || j >= njobs()) throw std::runtime_error("job index out of range");
+    uint32_t n = 0;
+    HIP_OK(hipMemcpy(&n, d_ndet_ + j, 4, hipMemcpyDeviceToHost));
+    return n;
+}
+
+uint32_t BatchSession::saved(int j) const { return std::min(detected(j), rows_.at(j).maxdet); }
+
+void BatchSession::fetch_detp(int j, float* rows, uint32_t* ids, float* weights) const {
+    HIP_OK(hipSetDevice(lc_.device));
+    const JobRows& jr = rows_.at(j);
+    const uint32_t n = saved(j);
+    if (rows && n && jr.detlen)
+        HIP_OK(hipMemcpy(rows, d_detp_ + jr.detp_off, (size_t)n * jr.detlen * 4, hipMemcpyDeviceToHost));
+    if (ids && n) HIP_OK(hipMemcpy(ids, d_detid_ + jr.row_off, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (weights && n) HIP_OK(hipMemcpy(weights, d_detw_ + jr.row_off, (size_t)n * 4, hipMemcpyDeviceToHost));
 }
 
 void BatchSession::energies(int j, double out[3]) const {

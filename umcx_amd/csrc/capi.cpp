@@ -391,6 +391,25 @@ int umcx_batch_energies(const umcx_batch* b, int32_t job, double out[3]) {
     return guarded([&]() -> int { batch_of(b, "umcx_batch_energies").energies(job, out); return 0; });
 }
 
+int umcx_batch_detected(const umcx_batch* b, int32_t job, uint32_t* detected, uint32_t* saved) {
+    return guarded([&]() -> int {
+        const umcx::BatchSession& bs = batch_of(b, "umcx_batch_detected");
+        if (job < 0 || job >= bs.njobs()) throw std::runtime_error("umcx_batch_detected: job index out of range");
+        if (detected) *detected = bs.detected(job);
+        if (saved) *saved = bs.saved(job);
+        return 0;
+    });
+}
+
+int umcx_batch_detp(const umcx_batch* b, int32_t job, float* rows, uint32_t* ids, float* weights) {
+    return guarded([&]() -> int {
+        const umcx::BatchSession& bs = batch_of(b, "umcx_batch_detp");
+        if (job < 0 || job >= bs.njobs()) throw std::runtime_error("umcx_batch_detp: job index out of range");
+        bs.fetch_detp(job, rows, ids, weights);
+        return 0;
+    });
+}
+
 int umcx_batch_trials(const umcx_batch* b, umcx_batch_trial* out, int32_t cap, int32_t* n) {
     return guarded([&]() -> int {
         const umcx::BatchSession& bs = batch_of(b, "umcx_batch_trials");
