@@ -31,6 +31,10 @@ namespace umcx {
 namespace dev {
 
 constexpr uint32_t kMaxBatchMedia = 128;   // labels travel as uint8 with the detector bit in bit 7
+// a wave takes new photons once this many of its lanes need one (or all do): the refill's cursor
+// atomic and launch code then serve two lanes at a time (256-job workload 9,977 -> 9,914 ms; four
+// lanes: 10,037 ms)
+constexpr int kBatchRefillMin = 2;
 
 // one job's constants (MCX_param :95-99 + source + seeds + buffer offsets + LDS tally box)
 struct __align__(16) JobDesc {
@@ -167,7 +171,7 @@ __global__ void __launch_bounds__(1024) batch_kernel(BatchArgs b) {
             const unsigned long long idle = __ballot(!alive);
             if (dry) {
                 if (idle == ~0ull) break;
-            } else if (idle) {
+            } else if (__popcll(idle) >= kBatchRefillMin || idle == ~0ull) {
                 const int leader = __ffsll((long long)idle) - 1;
                 unsigned long long base = 0;
                 if ((int)__lane_id() == leader) base = atomicAdd(b.jcur + job, (unsigned long long)__popcll(idle));
