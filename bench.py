@@ -381,7 +381,43 @@ def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3, target: fl
                                               "entries_per_s", "fd_plain_launch_ms", "speedup_vs_plain_loop",
                                               "fd_vs_replay_max_rel_diff", "noise_rel_std_median",
                                               "noise_rel_std_max")}
+    res["at_1pct_noise"]["jacobian_inclusion_noise"] = inclusion_jacobian_noise(n1)
     return res
+
+
+def inclusion_jacobian_noise(photons: int, batches: int = 8) -> dict:
+    """Noise of the Jacobian itself: d log I_sd / d mua summed over the inclusion's voxels, per
+    source-detector pair, from `batches` independent seed batches of photons/batches per source
+    (the sample std across batches / sqrt(batches) is the relative std at `photons` per source)."""
+    import copy
+
+    import numpy as np
+
+    from umcx_amd import jacobian as J
+    cfg = J.slab_config(nphoton=photons)
+    srcs = J.grid_sources(64, 64, 4, 16.5)
+    lab = np.asarray(__import__("umcx_amd").domain(cfg) & 0x7FFFFFFF).reshape(-1)
+    incl = np.flatnonzero(lab == 2)
+    vals = []
+    for b in range(batches):
+        c = copy.deepcopy(cfg)
+        c["Session"]["RNGSeed"] = 1648335518 + 7919 * (b + 1)
+        m = J.MultiSourceModel(c, srcs, photons // batches)
+        m.forward()
+        Jt = m.jacobian()
+        import torch
+        idx = torch.as_tensor(incl, device=Jt.device)
+        vals.append(Jt[:, idx, 0].double().sum(dim=1).cpu().numpy())
+        m.close()
+        del Jt
+    v = np.array(vals)                                         # (batches, pairs)
+    rel = v.std(axis=0, ddof=1) / np.abs(v.mean(axis=0)) / np.sqrt(batches)
+    log(f"bench: inclusion Jacobian noise at {photons:.3e} per source: median {np.median(rel):.4f}, max {rel.max():.4f}")
+    return {"quantity": "d log I_sd / d mua summed over the inclusion's 216 voxels, per pair",
+            "batches": batches, "photons_per_source_per_batch": photons // batches,
+            "noise_rel_std_median": float(np.median(rel)), "noise_rel_std_max": float(rel.max()),
+            "pairs_at_or_below_1pct": int((rel <= 0.01).sum()), "pairs": int(rel.size),
+            "photons_per_source_for_every_pair_at_1pct": int(np.ceil(photons * max(1.0, (rel.max() / 0.01) ** 2)))}
 
 
 def jacobian_at(sync, photons: int, fd_voxels: int) -> dict:
