@@ -199,6 +199,57 @@ def test_four_bit_lds_labels_match_oracle():
     assert np.array_equal(rows.view(np.uint32), sorted_rows(o["detp"]).view(np.uint32))
 
 
+def many_media_config(nmed: int, seed: int = 5):
+    """28 x 24 x 20 voxels in 4^3 blocks of `nmed` media (each block a random medium, refractive
+    indices 1.2-1.5, Fresnel on), pencil source, three detectors saving every partial path."""
+    rng = np.random.default_rng(seed)
+    dims = [28, 24, 20]
+    blocks = rng.integers(1, nmed + 1, size=(7, 6, 5))
+    vol = np.repeat(np.repeat(np.repeat(blocks, 4, 0), 4, 1), 4, 2).astype(np.int32)
+    media = [{"mua": 0.0, "mus": 0.0, "g": 1.0, "n": 1.0}]
+    for _ in range(nmed):
+        media.append({"mua": float(rng.uniform(0.0, 0.05)), "mus": float(rng.uniform(0.5, 4.0)),
+                      "g": float(rng.uniform(-0.5, 0.9)), "n": float(rng.uniform(1.2, 1.5))})
+    return {"Session": {"ID": f"media{nmed}", "Photons": 1500, "RNGSeed": 4711 + nmed, "DoMismatch": 1,
+                        "DoPartialPath": 1, "SaveDetFlag": 5, "OutputType": "x", "DoNormalize": 0},
+            "Forward": {"T0": 0, "T1": 2e-9, "Dt": 5e-10},
+            "Domain": {"Dim": dims, "LengthUnit": 1, "Media": media},
+            "Optode": {"Source": {"Type": "pencil", "Pos": [14, 12, 0], "Dir": [0, 0, 1]},
+                       "Detector": [{"Pos": [10, 12, 0], "R": 3}, {"Pos": [18, 12, 0], "R": 3},
+                                    {"Pos": [14, 6, 0], "R": 3}]},
+            "Shapes": {"_ArraySize_": dims, "_ArrayType_": "int32", "_ArrayData_": vol.reshape(-1).tolist()}}
+
+
+@pytest.mark.parametrize("nmed", [20, 140])
+def test_many_media_match_oracle(nmed):
+    """20 media: byte labels read from HBM (too many ids to pack), partial paths of 20 media per
+    lane in LDS; 140 media: 32-bit labels. Per-photon states, detected rows (with a partial path
+    per medium) and the fluence equal the oracle's."""
+    cfg = many_media_config(nmed)
+    prob = O.problem_from_config(cfg, U.domain(cfg))
+    n = cfg["Session"]["Photons"]
+    say(f"trace parity {nmed} media")
+    o = O.run(prob, nphoton=n, traces=True, nthreads=8)
+    with U.Simulation(cfg, trace=True) as sim:
+        assert sim.info["lds_label_bits"] == 0
+        assert sim.info["label_bytes"] == (4 if nmed > 128 else 1)
+        assert sim.info["ppath_columns"] == nmed + 1      # media 0..nmed (:157)
+        sim.run(0, n)
+        t = sim.trace(n)
+        rows = sorted_rows(sim.detp()[0])
+        flu = sim.fluence().astype(np.float64)
+    ot = o["traces"]
+    exact = lambda a: np.ascontiguousarray(a).view(np.uint32)
+    for key in ("pos", "vec", "len", "w0"):
+        assert np.array_equal(exact(t[key]), exact(ot[key])), key
+    assert np.array_equal(t["mediaid"], ot["mediaid"])
+    assert np.array_equal(t["lastvoxelidx"], ot["lastvoxelidx"])
+    assert len(rows) > 10 and rows.shape[1] == 2 + nmed
+    assert np.array_equal(rows.view(np.uint32), sorted_rows(o["detp"]).view(np.uint32))
+    of = np.asarray(o["fluence"], dtype=np.float64).reshape(flu.shape)
+    assert np.linalg.norm(flu - of) <= 1e-5 * np.linalg.norm(of)
+
+
 def test_unusual_refractive_indices_match_oracle():
     """Layers with refractive indices far from tissue values (0.45 and 4.5 beside 1.37): total
     internal reflection at most angles and large transmission bends; every photon matches the
