@@ -40,7 +40,7 @@ int umcx_device_count(void);
 typedef struct umcx_session umcx_session;
 
 /* Launch options. A NULL pointer takes the tuned defaults for every field. In a caller-filled
- * struct, block, blocks, label_cache and deposit_cache take their tuned defaults at 0, but
+ * struct, block, blocks, label_cache, deposit_cache and deposit_log take their tuned defaults at 0, but
  * tally_mode and vote must be -1 for theirs (0 selects global atomics only and immediate
  * scattering). No counterpart in the reference, whose ThreadNum/BlockSize session keys
  * (src/umcx.cpp:739-741) are honoured when block and blocks are zero. */
@@ -56,6 +56,15 @@ typedef struct umcx_launch {
                             umcx_info.lds_label_bits), -1: labels always read from HBM */
     int32_t deposit_cache; /* 0: per-lane cache of the last 4 global fluence deposits when the LDS tile
                               takes under half of them (pilot count), 1: always, -1: never */
+    int32_t deposit_log;   /* 0: global fluence deposits appended to a device log and summed into the
+                              grid by separate passes when the pilot counts at least 200 of them per
+                              photon (float atomics would saturate), 1: always, -1: never (float
+                              atomics); grids of 2^29 cells or more always use atomics */
+    int32_t deposit_log_segments; /* log capacity in segments of 4096 records, 64 KiB of device memory
+                                     each with the grouped copy (0: sized per run from the pilot's
+                                     count, at most a quarter of the free device memory); a run that
+                                     outgrows the log runs in chunks, and deposits past a full log
+                                     go to float atomics */
 } umcx_launch;
 
 /* Constants of a prepared problem: MCX_param (src/umcx.cpp:95-99) plus launch facts. */
@@ -75,6 +84,8 @@ typedef struct umcx_info {
                                          half the lanes */
     int32_t deposit_cache;            /* 1 when runs use the per-lane deposit cache */
     double deposit_cache_hits;        /* share of the pilot's global deposits that hit that cache */
+    int32_t deposit_log;              /* 1 when runs append global deposits to the deposit log */
+    double global_deposits_per_photon; /* the pilot's estimate for the runs (0 before a pilot) */
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)

@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libumcx.so")
 class UmcxLaunch(C.Structure):
     _fields_ = [("device", C.c_int32), ("block", C.c_int32), ("blocks", C.c_int32),
                 ("tally_mode", C.c_int32), ("trace", C.c_int32), ("vote", C.c_int32), ("label_cache", C.c_int32),
-                ("deposit_cache", C.c_int32)]
+                ("deposit_cache", C.c_int32), ("deposit_log", C.c_int32),
+                ("deposit_log_segments", C.c_int32)]
 
 
 class UmcxInfo(C.Structure):
@@ -27,7 +28,8 @@ class UmcxInfo(C.Structure):
                 ("unitinmm", C.c_float), ("seeds", C.c_uint32 * 4), ("block", C.c_int32), ("blocks", C.c_int32),
                 ("tally_mode", C.c_int32), ("label_bytes", C.c_int32), ("lds_label_bits", C.c_int32),
                 ("last_block", C.c_int32), ("last_blocks", C.c_int32), ("deposit_cache", C.c_int32),
-                ("deposit_cache_hits", C.c_double)]
+                ("deposit_cache_hits", C.c_double), ("deposit_log", C.c_int32),
+                ("global_deposits_per_photon", C.c_double)]
 
 
 class UmcxBatchTrial(C.Structure):

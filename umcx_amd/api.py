@@ -51,9 +51,11 @@ class Simulation:
 
     def __init__(self, cfg=None, argv: list[str] | None = None, device: int = 0, block: int = 0,
                  blocks: int = 0, tally_mode: int = -1, trace: bool = False, vote: int = -1, label_cache: int = 0,
-                 bjdata: bytes | None = None, deposit_cache: int = 0):
+                 bjdata: bytes | None = None, deposit_cache: int = 0, deposit_log: int = 0,
+                 deposit_log_segments: int = 0):
         L = lib()
-        launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote, label_cache, deposit_cache)
+        launch = UmcxLaunch(device, block, blocks, tally_mode, 1 if trace else 0, vote, label_cache, deposit_cache,
+                            deposit_log, deposit_log_segments)
         h = C.c_void_p()
         if bjdata is not None:      # a BJData config buffer, as MATLAB passes it (src/umcx.cpp:475-478)
             check(L.umcx_session_from_argv(buffer_argv(bjdata), -len(bjdata), C.byref(launch), C.byref(h)))
@@ -111,6 +113,19 @@ class Simulation:
         info = UmcxInfo()
         check(lib().umcx_session_info(self._h, C.byref(info)))
         return bool(info.deposit_cache)
+
+    def deposit_log(self) -> bool:
+        """Whether runs append global fluence deposits to the deposit log (summed into the grid by
+        separate passes) instead of issuing float atomics."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return bool(info.deposit_log)
+
+    def global_deposits_per_photon(self) -> float:
+        """The pilot's estimate of the global fluence deposits per photon of the runs (0 before a pilot)."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return float(info.global_deposits_per_photon)
 
     def deposit_cache_hits(self) -> float:
         """Share of the pilot's global deposits that hit the per-lane deposit cache (0 before a pilot)."""

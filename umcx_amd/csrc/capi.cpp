@@ -57,6 +57,8 @@ umcx::LaunchConfig to_launch(const umcx_launch* l) {
         lc.vote = l->vote;
         lc.label_cache = l->label_cache;
         lc.deposit_cache = l->deposit_cache;
+        lc.deposit_log = l->deposit_log;
+        lc.deposit_log_segments = l->deposit_log_segments;
     }
     return lc;
 }
@@ -220,6 +222,8 @@ int umcx_session_info(const umcx_session* s, umcx_info* info) {
         info->last_block = ss.last_block(); info->last_blocks = ss.last_blocks();
         info->deposit_cache = ss.deposit_cache() ? 1 : 0;
         info->deposit_cache_hits = ss.deposit_cache_hits();
+        info->deposit_log = ss.deposit_log() ? 1 : 0;
+        info->global_deposits_per_photon = ss.global_deposits_per_photon();
         return 0;
     });
 }

@@ -343,13 +343,13 @@ __device__ __forceinline__ bool boundary(Photon& p, float n1, float n2, Rng& r, 
 
 // Per-lane write-back cache of the last K global deposits, most recent first. A photon that
 // steps back into a voxel it left a moment ago (same time gate) adds to the cached partial sum
-// instead of issuing another memory-side atomic; an entry reaches the grid when it is evicted or
-// flushed. Only the float summation order of a cell changes. Returns true on a miss (a new entry).
-// Keys are 32-bit fluence indices (grids of fewer than 2^32 - 1 cells; the all-ones key marks an
-// empty entry).
+// instead of issuing another global deposit; an entry reaches the grid (through `emit(cell,
+// sum)`) when it is evicted or flushed. Only the float summation order of a cell changes. Returns
+// true on a miss (a new entry). Keys are 32-bit fluence indices (grids of fewer than 2^32 - 1
+// cells; the all-ones key marks an empty entry).
 constexpr uint32_t kNoKey = ~0u;
-template <int K>
-__device__ __forceinline__ bool cached_deposit(float* grid, uint32_t (&key)[K], float (&sum)[K], uint32_t idx,
+template <int K, typename Emit>
+__device__ __forceinline__ bool cached_deposit(Emit&& emit, uint32_t (&key)[K], float (&sum)[K], uint32_t idx,
                                                float val) {
     int pos = K;
 #pragma unroll
@@ -359,7 +359,7 @@ __device__ __forceinline__ bool cached_deposit(float* grid, uint32_t (&key)[K], 
 #pragma unroll
     for (int j = 0; j < K; j++) cur = pos == j ? in_reg(sum[j]) : cur;
     if (miss) {                             // the least recent entry goes to the grid
-        if (key[K - 1] != kNoKey) unsafeAtomicAdd(grid + key[K - 1], sum[K - 1]);
+        if (key[K - 1] != kNoKey) emit(key[K - 1], sum[K - 1]);
         pos = K - 1;
     }
 #pragma unroll
@@ -372,11 +372,11 @@ __device__ __forceinline__ bool cached_deposit(float* grid, uint32_t (&key)[K], 
     sum[0] = cur + val;
     return miss;
 }
-template <int K>
-__device__ __forceinline__ void flush_deposits(float* grid, uint32_t (&key)[K], float (&sum)[K]) {
+template <int K, typename Emit>
+__device__ __forceinline__ void flush_deposits(Emit&& emit, uint32_t (&key)[K], float (&sum)[K]) {
 #pragma unroll
     for (int j = 0; j < K; j++) {
-        if (key[j] != kNoKey) unsafeAtomicAdd(grid + key[j], sum[j]);
+        if (key[j] != kNoKey) emit(key[j], sum[j]);
         key[j] = kNoKey;
         sum[j] = 0.f;
     }

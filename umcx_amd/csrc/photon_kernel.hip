@@ -53,6 +53,18 @@ constexpr int kDepCache = 4;                       // global deposits cached per
 constexpr int kRefillMin = 2;
 constexpr double kCacheBelowShare = 0.5;           // ... used when the LDS tile takes less of the deposits
 constexpr double kCacheMinHits = 0.05;             // ... and when at least this share of the pilot's deposits hit it
+// Deposit log: sessions whose global deposits would saturate the memory-side atomics append them
+// as {cell, value} records instead, and separate passes sum the records into the grid.
+constexpr uint32_t kLogSeg = 4096;                 // records per log segment (32 KiB), one wave's at a time
+constexpr uint32_t kLogNone = 0xFFFFFFFFu;         // a wave that holds no segment yet
+constexpr uint32_t kLogFull = 0xFFFFFFFEu;         // the log is full: the wave's deposits go to atomics
+constexpr int kBucketBits = 15;                    // fluence cells per bucket of the reduce pass: 2^15 (128 KiB of LDS)
+constexpr uint32_t kMaxBuckets = 16384;            // buckets the histogram keeps in LDS (grids below 2^29 cells)
+constexpr double kLogMinRecords = 200.0;           // global deposits per photon from which the log is used
+constexpr int kLogGridPerCu = 2;                   // workgroups per CU of the histogram and scatter passes
+// the photon kernel's static LDS (slot counter, each wave's log segment and fill), kept out of the
+// dynamic LDS a launch may take
+constexpr size_t kStaticLds = 256;
 
 namespace dev {
 
@@ -107,6 +119,12 @@ struct KernelArgs {
     const int32_t* rsrc;
     const float* rsrcs;
     uint64_t msrc_n;
+    // deposit log: global deposits appended as {cell, value bits} to kLogSeg-record segments, a
+    // wave filling one segment at a time (null: global deposits are float atomics)
+    uint2* dlog;
+    uint32_t* dlog_fill;              // records written to each segment
+    unsigned long long* dlog_next;    // next free segment
+    uint32_t dlog_segs;               // segments in the log
 };
 
 // The kernel's arguments read again from the kernarg segment where they are used, through a
@@ -121,6 +139,39 @@ __device__ __forceinline__ ArgsRef reread_args() {
 __device__ __forceinline__ float4 ld4(const __attribute__((address_space(4))) float4* q) {
     const __attribute__((address_space(4))) float* f = (const __attribute__((address_space(4))) float*)q;
     return make_float4(f[0], f[1], f[2], f[3]);
+}
+
+// Append one global deposit per active lane to the deposit log. The lanes of a wave that deposit
+// together take consecutive records of the wave's current segment (one LDS read and write by
+// the first of them); a wave whose segment cannot take them records how far it filled it and
+// takes the next free segment from the device counter. Once the log is full the wave's deposits
+// go to the grid as atomics, so a run never loses one. s_seg/s_fill hold each wave's segment
+// and fill count in LDS, where every lane of the wave sees the latest values (volatile: the lane
+// that reads them may not be the one that wrote them last).
+__device__ __forceinline__ void log_deposit(volatile uint32_t* s_seg, volatile uint32_t* s_fill, uint32_t cell,
+                                            float val) {
+    const ArgsRef a = reread_args();
+    const unsigned long long act = __ballot(1);
+    const int leader = __ffsll((long long)act) - 1;
+    const uint32_t n = (uint32_t)__popcll(act), rank = (uint32_t)__popcll(act & lanemask_lt());
+    const int w = threadIdx.x >> 6;
+    uint32_t seg = 0, base = 0;
+    if ((int)__lane_id() == leader) {
+        seg = s_seg[w];
+        base = s_fill[w];
+        if (seg != kLogFull && base + n > kLogSeg) {
+            if (seg != kLogNone) a->dlog_fill[seg] = base;
+            const unsigned long long next = atomicAdd(a->dlog_next, 1ull);
+            seg = next < a->dlog_segs ? (uint32_t)next : kLogFull;
+            base = 0;
+        }
+        s_seg[w] = seg;
+        s_fill[w] = base + n;
+    }
+    seg = __shfl(seg, leader);
+    base = __shfl(base, leader);
+    if (seg != kLogFull) a->dlog[(uint64_t)seg * kLogSeg + base + rank] = make_uint2(cell, __float_as_uint(val));
+    else unsafeAtomicAdd(a->fluence + cell, val);
 }
 
 template <bool LDSLAB, typename LabelT>
@@ -157,11 +208,14 @@ __device__ __forceinline__ void set_props(Photon& p, const KernelArgs& a, const 
 
 // REPLAY: re-trace the photons listed in a.rids (photon i is a pure function of i) and add
 // each voxel segment's perturbation-MC terms to the gradient slices instead of tallying.
-template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY, bool LDSLAB>
+// DLOG: global deposits go to the deposit log (a.dlog) instead of float atomics.
+template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY, bool LDSLAB, bool DLOG = false>
 __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     // LDS: libm tables | media | packed labels (LDSLAB) | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
     __shared__ uint32_t s_taken;                                 // static photon slots handed out
+    __shared__ uint32_t s_lseg[16], s_lfill[16];                 // deposit log: each wave's segment and fill
+    static_assert(sizeof(uint32_t) * 33 <= kStaticLds, "static LDS exceeds its reserve");
     libm::LogfEntry* slog = (libm::LogfEntry*)lds4;            // 16 x 16 B
     uint64_t* sexp = (uint64_t*)(lds4 + 16);                     // 32 x 8 B
     float4* smedia = lds4 + 32;
@@ -172,6 +226,10 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     if (tid < 16) slog[tid] = libm::kLogfTab[tid];
     if (tid < 32) sexp[tid] = libm::kExp2Tab[tid];
     if (tid == 0) s_taken = 0u;
+    if (tid < 16) {
+        s_lseg[tid] = kLogNone;
+        s_lfill[tid] = kLogSeg;
+    }
     const Tabs tabs{slog, sexp};
     const Grid grid{a.nx, a.ny, a.nz, a.nyz};
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
@@ -209,6 +267,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     }
     uint32_t nglob = 0, nmiss = 0;   // deposit-cache statistics (reported by the pilot)
     const bool energy_out = a.outputtype == kEnergy;
+    // a global deposit: to the deposit log when the session keeps one, else a float atomic
+    auto emit = [&](uint32_t cell, float v) {
+        if (DLOG) log_deposit(s_lseg, s_lfill, cell, v);
+        else unsafeAtomicAdd(a.fluence + cell, v);
+    };
 
     while (true) {
         // ---- take a new photon when this lane's has ended (one atomic per wave) ----
@@ -335,9 +398,11 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
                         else if (a.dep_cache) {
                             nglob++;
-                            nmiss += cached_deposit(a.fluence, dkey, dsum, (uint32_t)p.lastvox + (uint32_t)tshift * (uint32_t)a.nxyz, val);
+                            nmiss += cached_deposit(emit, dkey, dsum, (uint32_t)p.lastvox + (uint32_t)tshift * (uint32_t)a.nxyz, val);
                         } else {
-                            unsafeAtomicAdd(a.fluence + (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz, val);
+                            const uint64_t cell = (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz;
+                            if (DLOG) log_deposit(s_lseg, s_lfill, (uint32_t)cell, val);   // cells < 2^32
+                            else unsafeAtomicAdd(a.fluence + cell, val);
                         }
                     }
                     p.lw = p.pw;
@@ -419,7 +484,12 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
         alive = false;
     }
 
-    if (savevol) flush_deposits(a.fluence, dkey, dsum);
+    if (savevol) flush_deposits(emit, dkey, dsum);
+    // the wave's last segment: how far it was filled
+    if (DLOG && __lane_id() == 0) {
+        const uint32_t seg = ((volatile uint32_t*)s_lseg)[tid >> 6];
+        if (seg != kLogNone && seg != kLogFull) reread_args()->dlog_fill[seg] = ((volatile uint32_t*)s_lfill)[tid >> 6];
+    }
     if (a.dep_stats) {
         const double g = wave_sum((double)nglob), m = wave_sum((double)nmiss);
         if (__lane_id() == 0) {
@@ -656,6 +726,106 @@ __global__ void libm_eval_kernel(int fn, const float* in, float* out, uint64_t n
     out[i] = y;
 }
 
+// ---- deposit-log passes: the log's records summed into the fluence grid ----
+// The cells are cut into buckets of 2^kBucketBits consecutive cells. log_hist_kernel counts each
+// workgroup's records per bucket, log_scan_kernel turns the counts into each (bucket, workgroup)
+// pair's first output slot, log_scatter_kernel writes the records grouped by bucket, and
+// log_reduce_kernel sums a slice of one bucket's records in LDS and adds the slice's nonzero
+// cells to the grid. hist and scatter take segments s = blockIdx.x, blockIdx.x + gridDim.x, ...
+// in the same way, so each workgroup writes exactly the slots it counted.
+__device__ __forceinline__ uint32_t log_segments(const unsigned long long* next, uint32_t segs) {
+    const unsigned long long n = *next;
+    return n < segs ? (uint32_t)n : segs;
+}
+
+__global__ void __launch_bounds__(1024) log_hist_kernel(const uint2* log, const uint32_t* fill,
+                                                        const unsigned long long* next, uint32_t segs, uint32_t nb,
+                                                        uint32_t* hist) {
+    extern __shared__ uint32_t lds_u32[];
+    uint32_t* h = lds_u32;
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    const uint32_t nseg = log_segments(next, segs);
+    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const uint32_t n = min(fill[sg], kLogSeg);
+        const uint2* r = log + (uint64_t)sg * kLogSeg;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&h[r[i].x >> kBucketBits], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * gridDim.x + blockIdx.x] = h[b];
+}
+
+// Exclusive prefix sums over hist (bucket-major: entry b * g + w) in place, one 1024-lane
+// workgroup; start[b] = the first slot of bucket b, start[nb] = the record count.
+__global__ void __launch_bounds__(1024) log_scan_kernel(uint32_t* hist, uint32_t nb, uint32_t g, uint32_t* start) {
+    __shared__ uint32_t part[1024];
+    const uint64_t e = (uint64_t)nb * g;
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    const uint64_t lo = e * t / nt, hi = e * (t + 1) / nt;
+    uint32_t sum = 0;
+    for (uint64_t i = lo; i < hi; i++) sum += hist[i];
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t o = 1; o < nt; o <<= 1) {
+        const uint32_t v = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+    for (uint64_t i = lo; i < hi; i++) {
+        const uint32_t c = hist[i];
+        hist[i] = run;
+        if (i % g == 0) start[i / g] = run;
+        run += c;
+    }
+    if (t == nt - 1) start[nb] = run;
+}
+
+__global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, const uint32_t* fill,
+                                                           const unsigned long long* next, uint32_t segs,
+                                                           uint32_t nb, const uint32_t* off, uint2* out) {
+    extern __shared__ uint32_t lds_u32[];
+    uint32_t* c = lds_u32;
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) c[b] = off[(uint64_t)b * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const uint32_t nseg = log_segments(next, segs);
+    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const uint32_t n = min(fill[sg], kLogSeg);
+        const uint2* r = log + (uint64_t)sg * kLogSeg;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const uint2 v = r[i];
+            out[atomicAdd(&c[v.x >> kBucketBits], 1u)] = v;
+        }
+    }
+}
+
+// Workgroup (b, j) of nb x k sums slice j of k of bucket b's records in LDS, then adds each
+// nonzero cell of the slice to the grid (several slices of a bucket add to the same cells).
+__global__ void __launch_bounds__(1024) log_reduce_kernel(const uint2* sorted, const uint32_t* start, uint32_t k,
+                                                          uint64_t ncells, float* grid) {
+    extern __shared__ float lds_f32[];
+    float* tile = lds_f32;
+    constexpr uint32_t kCells = 1u << kBucketBits;
+    const uint32_t b = blockIdx.x / k, j = blockIdx.x - b * k;
+    const uint32_t s0 = start[b], len = start[b + 1] - s0;
+    const uint32_t lo = s0 + (uint32_t)((uint64_t)len * j / k), hi = s0 + (uint32_t)((uint64_t)len * (j + 1) / k);
+    if (lo == hi) return;                    // the whole workgroup: no barrier is skipped by part of it
+    for (uint32_t i = threadIdx.x; i < kCells; i += blockDim.x) tile[i] = 0.f;
+    __syncthreads();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint2 v = sorted[i];
+        __hip_atomic_fetch_add(tile + (v.x & (kCells - 1)), __uint_as_float(v.y), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)b << kBucketBits;
+    for (uint32_t i = threadIdx.x; i < kCells; i += blockDim.x) {
+        const float v = tile[i];
+        if (v != 0.f && base + i < ncells) unsafeAtomicAdd(grid + base + i, v);
+    }
+}
+
 }  // namespace dev
 
 // ======================= host side =======================
@@ -664,29 +834,35 @@ namespace {
 
 using LaunchFn = void (*)(const dev::KernelArgs&, int, int, size_t, hipStream_t);
 
-template <bool R, bool D, typename L, bool P = false, bool S = false>
+template <bool R, bool D, typename L, bool P = false, bool S = false, bool G = false>
 void launch_one(const dev::KernelArgs& ka, int blocks, int block, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((dev::photon_kernel<R, D, L, P, S>), dim3(blocks), dim3(block), lds, s, ka);
+    hipLaunchKernelGGL((dev::photon_kernel<R, D, L, P, S, G>), dim3(blocks), dim3(block), lds, s, ka);
 }
 
-// kernel variants: reflection, detectors, label type, replay, labels in LDS (uint8 labels only)
-template <bool P>
+// kernel variants: reflection, detectors, label type, replay, labels in LDS (uint8 labels only),
+// deposit log (G)
+template <bool P, bool G = false>
 LaunchFn pick_variant(bool reflect, bool savedet, bool u8, bool lds) {
     if (u8 && lds) {
-        if (reflect) return savedet ? launch_one<true, true, uint8_t, P, true> : launch_one<true, false, uint8_t, P, true>;
-        return savedet ? launch_one<false, true, uint8_t, P, true> : launch_one<false, false, uint8_t, P, true>;
+        if (reflect)
+            return savedet ? launch_one<true, true, uint8_t, P, true, G> : launch_one<true, false, uint8_t, P, true, G>;
+        return savedet ? launch_one<false, true, uint8_t, P, true, G> : launch_one<false, false, uint8_t, P, true, G>;
     }
     if (u8) {
-        if (reflect) return savedet ? launch_one<true, true, uint8_t, P> : launch_one<true, false, uint8_t, P>;
-        return savedet ? launch_one<false, true, uint8_t, P> : launch_one<false, false, uint8_t, P>;
+        if (reflect)
+            return savedet ? launch_one<true, true, uint8_t, P, false, G> : launch_one<true, false, uint8_t, P, false, G>;
+        return savedet ? launch_one<false, true, uint8_t, P, false, G> : launch_one<false, false, uint8_t, P, false, G>;
     }
-    if (reflect) return savedet ? launch_one<true, true, uint32_t, P> : launch_one<true, false, uint32_t, P>;
-    return savedet ? launch_one<false, true, uint32_t, P> : launch_one<false, false, uint32_t, P>;
+    if (reflect)
+        return savedet ? launch_one<true, true, uint32_t, P, false, G> : launch_one<true, false, uint32_t, P, false, G>;
+    return savedet ? launch_one<false, true, uint32_t, P, false, G> : launch_one<false, false, uint32_t, P, false, G>;
 }
 
 LaunchFn pick_replay(bool reflect, bool u8, bool lds) { return pick_variant<true>(reflect, false, u8, lds); }
 
-LaunchFn pick(bool reflect, bool savedet, bool u8, bool lds) { return pick_variant<false>(reflect, savedet, u8, lds); }
+LaunchFn pick(bool reflect, bool savedet, bool u8, bool lds, bool dlog = false) {
+    return dlog ? pick_variant<false, true>(reflect, savedet, u8, lds) : pick_variant<false>(reflect, savedet, u8, lds);
+}
 
 template <bool R, bool D, typename L, bool S>
 const void* kptr() {
@@ -828,7 +1004,7 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     block_ = lc.block > 0 ? lc.block : (plabel_words_ ? 1024 : (p.blocksize > 0 ? p.blocksize : 256));
     block_ = std::max(64, std::min(1024, (block_ / 64) * 64));
     size_t lds = lds_bytes(0);
-    const size_t lds_wg_max = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 64 * 1024;
+    const size_t lds_wg_max = (prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 64 * 1024) - kStaticLds;
     while (lds > lds_wg_max && block_ > 64) {
         block_ /= 2;
         lds = lds_bytes(0);
@@ -844,7 +1020,7 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     // LDS tally tile: share the CU's LDS between the workgroups the register budget lets in
     tally_mode_ = lc.tally_mode;
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 160 * 1024;
-    per_wg_lds_ = std::min(lds_cu / (size_t)per_cu, lds_wg_max);
+    per_wg_lds_ = std::min(lds_cu / (size_t)per_cu - kStaticLds, lds_wg_max);
     tile_budget_ = per_wg_lds_ > lds ? std::min<uint64_t>((per_wg_lds_ - lds) / 4, kMaxTileCells) : 0;
     if (tally_mode_ == 1 && tile_budget_ < 64) tile_budget_ = 64;
     blocks_ = lc.blocks > 0 ? lc.blocks : prop.multiProcessorCount * per_cu;
@@ -858,7 +1034,9 @@ Session::~Session() {
     (void)hipSetDevice(lc_.device);
     for (void* ptr : {(void*)d_labels_, (void*)d_media_, (void*)d_detpos_, (void*)d_fluence_, (void*)d_detp_,
                       (void*)d_detid_, (void*)d_detw_, (void*)d_counters_, (void*)d_cursor_, (void*)d_energy_,
-                      (void*)d_trace_, d_vprops_, (void*)d_plabels_, (void*)d_plabels1_, (void*)d_msrc_})
+                      (void*)d_trace_, d_vprops_, (void*)d_plabels_, (void*)d_plabels1_, (void*)d_msrc_, d_dlog_,
+                      d_dlog_sorted_, (void*)d_dlog_fill_, (void*)d_dlog_next_, (void*)d_dlog_hist_,
+                      (void*)d_dlog_start_})
         if (ptr) (void)hipFree(ptr);
     if (ev0_) (void)hipEventDestroy((hipEvent_t)ev0_);
     if (ev1_) (void)hipEventDestroy((hipEvent_t)ev1_);
@@ -1003,6 +1181,67 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     // the per-lane deposit cache pays where most deposits go to HBM and a fair share of them hit
     // it (the pilot ran with it); otherwise its extra work per deposit costs more than it saves
     dep_cache_ = (total > 0 ? in_box / total : 0.0) < kCacheBelowShare && dep_hits_ >= kCacheMinHits;
+    // global deposits per photon the runs will issue (outside the tile, after the cache): past
+    // kLogMinRecords they would saturate the memory-side atomics, and the deposit log takes them
+    const double share_out = total > 0 ? 1.0 - in_box / total : 1.0;
+    dlog_rpp_ = cache ? (double)head[2] / (double)pilot * share_out * (deposit_cache() ? 1.0 - dep_hits_ : 1.0) : 0.0;
+    dlog_auto_ = dlog_rpp_ >= kLogMinRecords;
+}
+
+bool Session::deposit_log() const {
+    if (!prob_.issavevol || fluence_len_ > ((uint64_t)kMaxBuckets << kBucketBits)) return false;
+    return lc_.deposit_log > 0 || (lc_.deposit_log == 0 && dlog_auto_);
+}
+
+// Log buffers for a run of `count` photons: as many records as the pilot expects (times 1.25),
+// at least four partly filled segments' worth per wave, at most 2^31 records and a quarter of the
+// free device memory for the log and its grouped copy together; runs that need more are split
+// into chunks. An explicit capacity (LaunchConfig::deposit_log_segments) is taken as given.
+void Session::ensure_log(uint64_t count) {
+    const double rpp = dlog_rpp_ > 0.0 ? dlog_rpp_ : 512.0;
+    const uint64_t waves = (uint64_t)blocks_ * (uint64_t)((block_ + 63) / 64);
+    uint64_t want;
+    if (lc_.deposit_log_segments > 0) {
+        want = (uint64_t)lc_.deposit_log_segments * kLogSeg;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b));
+        const uint64_t cap = std::min<uint64_t>(1ull << 31, (uint64_t)(free_b + 16 * dlog_records_) / 4 / 16);
+        want = std::max<uint64_t>((uint64_t)((double)count * rpp * 1.25), 4 * waves * kLogSeg);
+        want = std::min(want, cap) / kLogSeg * kLogSeg;
+        if (want < kLogSeg) throw std::runtime_error("no device memory left for the deposit log");
+    }
+    if (dlog_records_ >= want) return;
+    for (void* ptr : {d_dlog_, d_dlog_sorted_, (void*)d_dlog_fill_, (void*)d_dlog_next_, (void*)d_dlog_hist_,
+                      (void*)d_dlog_start_})
+        if (ptr) HIP_OK(hipFree(ptr));
+    const uint32_t nb = (uint32_t)((fluence_len_ + (1ull << kBucketBits) - 1) >> kBucketBits);
+    d_dlog_ = dalloc<uint2>(want);
+    d_dlog_sorted_ = dalloc<uint2>(want);
+    d_dlog_fill_ = dalloc<uint32_t>(want / kLogSeg);
+    d_dlog_next_ = dalloc<unsigned long long>(1);
+    d_dlog_hist_ = dalloc<uint32_t>((size_t)nb * kLogGridPerCu * cu_count_);
+    d_dlog_start_ = dalloc<uint32_t>(nb + 1);
+    dlog_records_ = want;
+}
+
+// The passes that add the log's records to the fluence grid, enqueued after a photon launch.
+void Session::enqueue_log_passes(void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t nb = (uint32_t)((fluence_len_ + (1ull << kBucketBits) - 1) >> kBucketBits);
+    const uint32_t segs = (uint32_t)(dlog_records_ / kLogSeg);
+    const int g = kLogGridPerCu * cu_count_;
+    const size_t hist_lds = (size_t)nb * 4;
+    hipLaunchKernelGGL(dev::log_hist_kernel, dim3(g), dim3(256), hist_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
+                       d_dlog_next_, segs, nb, d_dlog_hist_);
+    hipLaunchKernelGGL(dev::log_scan_kernel, dim3(1), dim3(1024), 0, s, d_dlog_hist_, nb, (uint32_t)g, d_dlog_start_);
+    hipLaunchKernelGGL(dev::log_scatter_kernel, dim3(g), dim3(256), hist_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
+                       d_dlog_next_, segs, nb, d_dlog_hist_, (uint2*)d_dlog_sorted_);
+    // several workgroups per bucket when there are fewer buckets than four per CU
+    const uint32_t k = std::max<uint32_t>(1u, (uint32_t)((4ull * cu_count_ + nb - 1) / nb));
+    hipLaunchKernelGGL(dev::log_reduce_kernel, dim3(nb * k), dim3(1024), (size_t)4 << kBucketBits, s,
+                       (const uint2*)d_dlog_sorted_, d_dlog_start_, k, fluence_len_, d_fluence_);
+    HIP_OK(hipGetLastError());
 }
 
 // Launch shape of one run. The full shape fills every CU; a run of fewer than
@@ -1053,10 +1292,52 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     last_blocks_ = rblocks;
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
     size_t lds = lds_bytes(ka.tile_n);
+    const LaunchFn launch = pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0);
+    if (deposit_log()) {
+        const LaunchFn launch_log = pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0, true);
+        // global deposits go to the log; a run larger than the log holds runs in chunks, each
+        // launch followed by the passes that add its records to the grid
+        ensure_log(count);
+        const double rpp = dlog_rpp_ > 0.0 ? dlog_rpp_ : 512.0;
+        const uint64_t waves = (uint64_t)rblocks * (uint64_t)((rblock + 63) / 64);
+        // a wave may leave up to two segments partly filled
+        const uint64_t room = dlog_records_ > 2 * waves * kLogSeg ? dlog_records_ - 2 * waves * kLogSeg : 0;
+        uint64_t per_chunk = std::max<uint64_t>(1, (uint64_t)((double)room / (rpp * 1.1)));
+        // an explicit capacity: at most four chunks (deposits past a full log go to atomics)
+        if (lc_.deposit_log_segments > 0) per_chunk = std::max<uint64_t>(per_chunk, (count + 3) / 4);
+        HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
+        for (uint64_t done = 0; done < count;) {
+            const uint64_t n = std::min(per_chunk, count - done);
+            dev::KernelArgs kc = ka;
+            kc.first = first + done;
+            kc.count = n;
+            int cblock = block_, cblocks = blocks_;
+            run_shape(n, &cblock, &cblocks);
+            kc.per_wg = (n - n / 8) / (uint64_t)cblocks;
+            if (kc.trace) kc.trace = ka.trace + done;   // trace entries are per photon of the run
+            kc.dlog = (uint2*)d_dlog_;
+            kc.dlog_fill = d_dlog_fill_;
+            kc.dlog_next = d_dlog_next_;
+            kc.dlog_segs = (uint32_t)(dlog_records_ / kLogSeg);
+            hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
+            hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_dlog_next_, 1);
+            HIP_OK(hipGetLastError());
+            launch_log(kc, cblocks, cblock, lds_bytes(kc.tile_n), s);
+            HIP_OK(hipGetLastError());
+            enqueue_log_passes(s);
+            done += n;
+        }
+        HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
+        HIP_OK(hipStreamSynchronize(s));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, (hipEvent_t)ev0_, (hipEvent_t)ev1_));
+        st.kernel_ms = ms;
+        return st;
+    }
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0)(ka, rblocks, rblock, lds, s);
+    launch(ka, rblocks, rblock, lds, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -1093,6 +1374,8 @@ void Session::set_voxel_props(const float* mua, const float* mus, bool on_device
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
     dep_cache_ = true;
+    dlog_auto_ = false;
+    dlog_rpp_ = 0.0;
 }
 
 void Session::set_sources(const float* table, int nsrc, uint64_t per_source) {
@@ -1111,6 +1394,8 @@ void Session::set_sources(const float* table, int nsrc, uint64_t per_source) {
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
     dep_cache_ = true;
+    dlog_auto_ = false;
+    dlog_rpp_ = 0.0;
 }
 
 void Session::set_source(const float pos[3], const float dir[3]) {
@@ -1121,6 +1406,8 @@ void Session::set_source(const float pos[3], const float dir[3]) {
     tile_chosen_ = false;
     for (int& v : tile_) v = 0;
     dep_cache_ = true;
+    dlog_auto_ = false;
+    dlog_rpp_ = 0.0;
 }
 
 RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float* coef, uint64_t n, float* gmua,
@@ -1258,7 +1545,7 @@ const NoAllocDevice& noalloc_device(int dev) {
         HIP_OK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         d.ptr = (unsigned char*)ptr;
         d.cus = std::max(1, cus);
-        d.lds_wg_max = lds > 0 ? (size_t)lds : 64 * 1024;
+        d.lds_wg_max = (lds > 0 ? (size_t)lds : 64 * 1024) - kStaticLds;
         d.ready = true;
     }
     return d;

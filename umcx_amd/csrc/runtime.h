@@ -21,6 +21,9 @@ struct LaunchConfig {
     int vote = -1;           // scatter-round threshold in lanes (0: scatter right away, -1: tuned)
     int label_cache = 0;     // 0: labels in LDS when they fit, -1: labels read from HBM
     int deposit_cache = 0;   // 0: chosen from the pilot, 1: always, -1: never
+    int deposit_log = 0;     // global deposits to a log summed by separate passes: 0 chosen from the
+                             // pilot, 1 always (grids below 2^29 cells), -1 never
+    int deposit_log_segments = 0;   // log capacity in segments of 4096 records (0: sized per run)
     bool trace = false;      // keep each photon's final state (parity tests; small runs only)
 };
 
@@ -85,6 +88,10 @@ public:
         return fluence_len_ < 0xFFFFFFFFull && (lc_.deposit_cache > 0 || (lc_.deposit_cache == 0 && dep_cache_));
     }
     double deposit_cache_hits() const { return dep_hits_; }   // from the last pilot (0 before one)
+    // global deposits go to the deposit log (summed into the grid by separate passes) instead of
+    // float atomics: chosen by the pilot when they would saturate the memory-side atomics
+    bool deposit_log() const;
+    double global_deposits_per_photon() const { return dlog_rpp_; }   // pilot estimate (0 before one)
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
     float* device_fluence() const { return d_fluence_; }
@@ -130,6 +137,19 @@ private:
     uint32_t* d_plabels1_ = nullptr;
     uint32_t plabel1_words_ = 0, plabel1_id0_ = 0, plabel1_step_ = 0;
     size_t per_wg_lds_ = 0;     // LDS per workgroup at the session's launch shape
+    // deposit log: records {cell, value}, the same records grouped by bucket, each segment's
+    // fill, the segment counter, the (bucket, workgroup) counts and the bucket starts
+    bool dlog_auto_ = false;             // the pilot chose the log
+    double dlog_rpp_ = 0.0;              // global deposits per photon expected from the pilot
+    void* d_dlog_ = nullptr;
+    void* d_dlog_sorted_ = nullptr;
+    uint32_t* d_dlog_fill_ = nullptr;
+    unsigned long long* d_dlog_next_ = nullptr;
+    uint32_t* d_dlog_hist_ = nullptr;
+    uint32_t* d_dlog_start_ = nullptr;
+    uint64_t dlog_records_ = 0;          // capacity in records (a multiple of the segment size)
+    void ensure_log(uint64_t count);
+    void enqueue_log_passes(void* stream);
 
     void* build_args(uint64_t first, uint64_t count, void* out) const;
     size_t lds_bytes(uint32_t tile_n, bool paths_needed = true) const;   // dynamic LDS of a launch
