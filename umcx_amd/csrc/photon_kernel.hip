@@ -58,8 +58,8 @@ constexpr double kCacheMinHits = 0.05;             // ... and when at least this
 constexpr uint32_t kLogSeg = 4096;                 // records per log segment (32 KiB), one wave's at a time
 constexpr uint32_t kLogNone = 0xFFFFFFFFu;         // a wave that holds no segment yet
 constexpr uint32_t kLogFull = 0xFFFFFFFEu;         // the log is full: the wave's deposits go to atomics
-constexpr int kBucketBits = 15;                    // fluence cells per bucket of the reduce pass: 2^15 (128 KiB of LDS)
-constexpr uint32_t kMaxBuckets = 16384;            // buckets the histogram keeps in LDS (grids below 2^29 cells)
+constexpr int kBucketBits = 14;                    // fluence cells per bucket of the reduce pass: 2^14 (64 KiB of LDS)
+constexpr uint32_t kMaxBuckets = 4096;             // buckets of the log passes' LDS counters (grids below 2^26 cells)
 constexpr double kLogMinRecords = 200.0;           // global deposits per photon from which the log is used
 constexpr int kLogGridPerCu = 2;                   // workgroups per CU of the histogram and scatter passes
 // the photon kernel's static LDS (slot counter, each wave's log segment and fill), kept out of the
@@ -782,21 +782,74 @@ __global__ void __launch_bounds__(1024) log_scan_kernel(uint32_t* hist, uint32_t
     if (t == nt - 1) start[nb] = run;
 }
 
+// One segment at a time, counting-sorted by bucket in LDS so that each bucket's records leave as
+// one run that consecutive lanes write to consecutive slots (scattered 8-byte writes had kept the
+// pass at a fifth of its bandwidth): ranks within the segment from LDS counters, the buckets'
+// first staged positions from a prefix sum, the records staged in bucket order, then written to
+// the workgroup's next slots in their buckets. 1024 lanes, four records each.
 __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, const uint32_t* fill,
                                                            const unsigned long long* next, uint32_t segs,
                                                            uint32_t nb, const uint32_t* off, uint2* out) {
     extern __shared__ uint32_t lds_u32[];
-    uint32_t* c = lds_u32;
-    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) c[b] = off[(uint64_t)b * gridDim.x + blockIdx.x];
-    __syncthreads();
+    uint32_t* gofs = lds_u32;                          // nb: the workgroup's next slot in each bucket
+    uint32_t* cnt = gofs + nb;                         // nb: the segment's records per bucket
+    uint32_t* lst = cnt + nb;                          // nb: first staged position of each bucket
+    uint2* stage = (uint2*)(lst + nb + (nb & 1u));     // kLogSeg records, 8-byte aligned
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t b = t; b < nb; b += 1024) gofs[b] = off[(uint64_t)b * gridDim.x + blockIdx.x];
+    const uint32_t per = (nb + 1023) / 1024;           // counters per lane in the prefix sum
+    const uint32_t b0 = min(t * per, nb), b1 = min(b0 + per, nb);
     const uint32_t nseg = log_segments(next, segs);
     for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
         const uint32_t n = min(fill[sg], kLogSeg);
         const uint2* r = log + (uint64_t)sg * kLogSeg;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint2 v = r[i];
-            out[atomicAdd(&c[v.x >> kBucketBits], 1u)] = v;
+        for (uint32_t b = t; b < nb; b += 1024) cnt[b] = 0u;
+        __syncthreads();
+        uint2 v[4];
+        uint32_t rk[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = t + k * 1024;
+            v[k] = make_uint2(0u, 0u);
+            rk[k] = 0u;
+            if (i < n) {
+                v[k] = r[i];
+                rk[k] = atomicAdd(&cnt[v[k].x >> kBucketBits], 1u);
+            }
         }
+        __syncthreads();
+        uint32_t sum = 0;
+        for (uint32_t b = b0; b < b1; b++) sum += cnt[b];
+        uint32_t x = sum;                              // inclusive scan over the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((int)__lane_id() >= o) x += y;
+        }
+        if (__lane_id() == 63) wsum[t >> 6] = x;
+        __syncthreads();
+        uint32_t run = x - sum;
+        for (uint32_t w = 0; w < (t >> 6); w++) run += wsum[w];
+        for (uint32_t b = b0; b < b1; b++) {
+            lst[b] = run;
+            run += cnt[b];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (t + k * 1024 < n) stage[lst[v[k].x >> kBucketBits] + rk[k]] = v[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t i = t + k * 1024;
+            if (i < n) {
+                const uint2 w = stage[i];
+                const uint32_t b = w.x >> kBucketBits;
+                out[gofs[b] + (i - lst[b])] = w;
+            }
+        }
+        __syncthreads();
+        for (uint32_t b = t; b < nb; b += 1024) gofs[b] += cnt[b];
     }
 }
 
@@ -813,7 +866,19 @@ __global__ void __launch_bounds__(1024) log_reduce_kernel(const uint2* sorted, c
     if (lo == hi) return;                    // the whole workgroup: no barrier is skipped by part of it
     for (uint32_t i = threadIdx.x; i < kCells; i += blockDim.x) tile[i] = 0.f;
     __syncthreads();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    // eight records per lane in flight before their LDS adds
+    constexpr uint32_t kUnroll = 8;
+    uint32_t i = lo + threadIdx.x;
+    for (; i + (kUnroll - 1) * blockDim.x < hi; i += kUnroll * blockDim.x) {
+        uint2 v[kUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; u++) v[u] = sorted[i + u * blockDim.x];
+#pragma unroll
+        for (uint32_t u = 0; u < kUnroll; u++)
+            __hip_atomic_fetch_add(tile + (v[u].x & (kCells - 1)), __uint_as_float(v[u].y), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    for (; i < hi; i += blockDim.x) {
         const uint2 v = sorted[i];
         __hip_atomic_fetch_add(tile + (v.x & (kCells - 1)), __uint_as_float(v.y), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1232,10 +1297,11 @@ void Session::enqueue_log_passes(void* stream) {
     const uint32_t segs = (uint32_t)(dlog_records_ / kLogSeg);
     const int g = kLogGridPerCu * cu_count_;
     const size_t hist_lds = (size_t)nb * 4;
-    hipLaunchKernelGGL(dev::log_hist_kernel, dim3(g), dim3(256), hist_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
+    hipLaunchKernelGGL(dev::log_hist_kernel, dim3(g), dim3(1024), hist_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
                        d_dlog_next_, segs, nb, d_dlog_hist_);
     hipLaunchKernelGGL(dev::log_scan_kernel, dim3(1), dim3(1024), 0, s, d_dlog_hist_, nb, (uint32_t)g, d_dlog_start_);
-    hipLaunchKernelGGL(dev::log_scatter_kernel, dim3(g), dim3(256), hist_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
+    const size_t scatter_lds = ((size_t)3 * nb + (nb & 1u)) * 4 + (size_t)kLogSeg * 8;
+    hipLaunchKernelGGL(dev::log_scatter_kernel, dim3(g), dim3(1024), scatter_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
                        d_dlog_next_, segs, nb, d_dlog_hist_, (uint2*)d_dlog_sorted_);
     // several workgroups per bucket when there are fewer buckets than four per CU
     const uint32_t k = std::max<uint32_t>(1u, (uint32_t)((4ull * cu_count_ + nb - 1) / nb));

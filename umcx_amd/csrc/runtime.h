@@ -22,7 +22,7 @@ struct LaunchConfig {
     int label_cache = 0;     // 0: labels in LDS when they fit, -1: labels read from HBM
     int deposit_cache = 0;   // 0: chosen from the pilot, 1: always, -1: never
     int deposit_log = 0;     // global deposits to a log summed by separate passes: 0 chosen from the
-                             // pilot, 1 always (grids below 2^29 cells), -1 never
+                             // pilot, 1 always (grids below 2^26 cells), -1 never
     int deposit_log_segments = 0;   // log capacity in segments of 4096 records (0: sized per run)
     bool trace = false;      // keep each photon's final state (parity tests; small runs only)
 };
