@@ -168,8 +168,8 @@ __device__ __forceinline__ void log_deposit(volatile uint32_t* s_seg, volatile u
         s_seg[w] = seg;
         s_fill[w] = base + n;
     }
-    seg = __shfl(seg, leader);
-    base = __shfl(base, leader);
+    seg = (uint32_t)__builtin_amdgcn_readlane((int)seg, leader);
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
     if (seg != kLogFull) a->dlog[(uint64_t)seg * kLogSeg + base + rank] = make_uint2(cell, __float_as_uint(val));
     else unsafeAtomicAdd(a->fluence + cell, val);
 }
