@@ -1254,7 +1254,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
 }
 
 bool Session::deposit_log() const {
-    if (!prob_.issavevol || fluence_len_ > ((uint64_t)kMaxBuckets << kBucketBits)) return false;
+    if (!prob_.issavevol || dlog_unavailable_ || fluence_len_ > ((uint64_t)kMaxBuckets << kBucketBits)) return false;
     return lc_.deposit_log > 0 || (lc_.deposit_log == 0 && dlog_auto_);
 }
 
@@ -1288,6 +1288,26 @@ void Session::ensure_log(uint64_t count) {
     d_dlog_hist_ = dalloc<uint32_t>((size_t)nb * kLogGridPerCu * cu_count_);
     d_dlog_start_ = dalloc<uint32_t>(nb + 1);
     dlog_records_ = want;
+}
+
+// The log for a run, or false when the device cannot hold it (the run then uses float atomics,
+// with the same results up to summation order).
+bool Session::log_ready(uint64_t count) {
+    try {
+        ensure_log(count);
+        return true;
+    } catch (const std::exception&) {
+        for (void* ptr : {d_dlog_, d_dlog_sorted_, (void*)d_dlog_fill_, (void*)d_dlog_next_, (void*)d_dlog_hist_,
+                          (void*)d_dlog_start_})
+            if (ptr) (void)hipFree(ptr);
+        d_dlog_ = d_dlog_sorted_ = nullptr;
+        d_dlog_fill_ = d_dlog_hist_ = d_dlog_start_ = nullptr;
+        d_dlog_next_ = nullptr;
+        dlog_records_ = 0;
+        dlog_unavailable_ = true;  // the session stays on float atomics
+        (void)hipGetLastError();   // an allocation failure leaves no lasting error state
+        return false;
+    }
 }
 
 // The passes that add the log's records to the fluence grid, enqueued after a photon launch.
@@ -1359,11 +1379,10 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
     size_t lds = lds_bytes(ka.tile_n);
     const LaunchFn launch = pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0);
-    if (deposit_log()) {
+    if (deposit_log() && log_ready(count)) {
         const LaunchFn launch_log = pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0, true);
         // global deposits go to the log; a run larger than the log holds runs in chunks, each
         // launch followed by the passes that add its records to the grid
-        ensure_log(count);
         const double rpp = dlog_rpp_ > 0.0 ? dlog_rpp_ : 512.0;
         const uint64_t waves = (uint64_t)rblocks * (uint64_t)((rblock + 63) / 64);
         // a wave may leave up to two segments partly filled

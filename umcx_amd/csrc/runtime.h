@@ -140,6 +140,7 @@ private:
     // deposit log: records {cell, value}, the same records grouped by bucket, each segment's
     // fill, the segment counter, the (bucket, workgroup) counts and the bucket starts
     bool dlog_auto_ = false;             // the pilot chose the log
+    bool dlog_unavailable_ = false;      // the device could not hold a log: float atomics
     double dlog_rpp_ = 0.0;              // global deposits per photon expected from the pilot
     void* d_dlog_ = nullptr;
     void* d_dlog_sorted_ = nullptr;
@@ -149,6 +150,7 @@ private:
     uint32_t* d_dlog_start_ = nullptr;
     uint64_t dlog_records_ = 0;          // capacity in records (a multiple of the segment size)
     void ensure_log(uint64_t count);
+    bool log_ready(uint64_t count);
     void enqueue_log_passes(void* stream);
 
     void* build_args(uint64_t first, uint64_t count, void* out) const;
