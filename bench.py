@@ -515,7 +515,9 @@ def other_configs_leg(sync) -> dict:
     """Photons/ms of the other single-GPU configurations: the small homogeneous cube (60^3,
     reflection off, 1e6 photons, no detectors), the reference's published cube60 benchmark
     (the same cube with its 4 detectors, 1e8 photons) and the layered skin model with a vessel
-    (200^3 voxels of 5 um, Fresnel on, disk source r=60 voxels, 1e8 photons, energy output)."""
+    (200^3 voxels of 5 um, Fresnel on, disk source r=60 voxels, 1e8 photons, energy output; its
+    pilot counts about 450 global deposits per photon, so its session sums them through the
+    deposit log instead of float atomics)."""
     import umcx_amd as U
     out = {}
     cube = U.config_from_argv(["--bench", "cube60", "-d", "0", "-n", "1000000"])
@@ -535,6 +537,8 @@ def other_configs_leg(sync) -> dict:
             out[name] = {"photons": n, "kernel_ms": round(kms, 3), "photons_per_ms": round(n / kms, 1),
                          "lds_label_bits": sim.info["lds_label_bits"], "tally_box": sim.tally_box(),
                          "deposit_cache": sim.deposit_cache(), "deposit_cache_pilot_hits": round(sim.deposit_cache_hits(), 4),
+                         "deposit_log": sim.deposit_log(),
+                         "global_deposits_per_photon": round(sim.global_deposits_per_photon(), 1),
                          "absorbed_percent": (tot - esc) / tot * 100.0}
             if name == "cube60_published_1e8":
                 # webpage/umcx.html:451,455 of the reference: 175,075 photons/ms on an RTX 5090
