@@ -58,6 +58,7 @@ constexpr double kCacheMinHits = 0.05;             // ... and when at least this
 constexpr uint32_t kLogSeg = 4096;                 // records per log segment (32 KiB), one wave's at a time
 constexpr uint32_t kLogNone = 0xFFFFFFFFu;         // a wave that holds no segment yet
 constexpr uint32_t kLogFull = 0xFFFFFFFEu;         // the log is full: the wave's deposits go to atomics
+constexpr uint32_t kTileSegs = 2;                 // log segments the scatter pass sorts together
 constexpr int kBucketBits = 14;                    // fluence cells per bucket of the reduce pass: 2^14 (64 KiB of LDS)
 constexpr uint32_t kMaxBuckets = 4096;             // buckets of the log passes' LDS counters (grids below 2^26 cells)
 constexpr double kLogMinRecords = 200.0;           // global deposits per photon from which the log is used
@@ -746,11 +747,12 @@ __global__ void __launch_bounds__(1024) log_hist_kernel(const uint2* log, const 
     for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) h[b] = 0u;
     __syncthreads();
     const uint32_t nseg = log_segments(next, segs);
-    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-        const uint32_t n = min(fill[sg], kLogSeg);
-        const uint2* r = log + (uint64_t)sg * kLogSeg;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&h[r[i].x >> kBucketBits], 1u);
-    }
+    for (uint32_t tile = blockIdx.x; tile * kTileSegs < nseg; tile += gridDim.x)
+        for (uint32_t sg = tile * kTileSegs; sg < min((tile + 1) * kTileSegs, nseg); sg++) {
+            const uint32_t n = min(fill[sg], kLogSeg);
+            const uint2* r = log + (uint64_t)sg * kLogSeg;
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&h[r[i].x >> kBucketBits], 1u);
+        }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) hist[(uint64_t)b * gridDim.x + blockIdx.x] = h[b];
 }
@@ -782,39 +784,55 @@ __global__ void __launch_bounds__(1024) log_scan_kernel(uint32_t* hist, uint32_t
     if (t == nt - 1) start[nb] = run;
 }
 
-// One segment at a time, counting-sorted by bucket in LDS so that each bucket's records leave as
-// one run that consecutive lanes write to consecutive slots (scattered 8-byte writes had kept the
-// pass at a fifth of its bandwidth): ranks within the segment from LDS counters, the buckets'
-// first staged positions from a prefix sum, the records staged in bucket order, then written to
-// the workgroup's next slots in their buckets. 1024 lanes, four records each.
+// A tile of kTileSegs segments at a time, counting-sorted by bucket in LDS so that each bucket's
+// records leave as one run that consecutive lanes write to consecutive slots (scattered 8-byte
+// writes had kept the pass at a fifth of its bandwidth): ranks within the tile from LDS counters,
+// the buckets' first staged positions from a prefix sum, the records staged in bucket order, then
+// written to the workgroup's next slots in their buckets. 1024 lanes, kTileRecs / 1024 records
+// each. Tiles are the histogram's: workgroup w takes tiles w, w + gridDim.x, ...
+constexpr uint32_t kTileRecs = kTileSegs * kLogSeg;
+constexpr uint32_t kPerLane = kTileRecs / 1024;
+static_assert(kTileRecs % 1024 == 0, "a tile is a whole number of records per lane");
 __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, const uint32_t* fill,
                                                            const unsigned long long* next, uint32_t segs,
                                                            uint32_t nb, const uint32_t* off, uint2* out) {
     extern __shared__ uint32_t lds_u32[];
     uint32_t* gofs = lds_u32;                          // nb: the workgroup's next slot in each bucket
-    uint32_t* cnt = gofs + nb;                         // nb: the segment's records per bucket
+    uint32_t* cnt = gofs + nb;                         // nb: the tile's records per bucket
     uint32_t* lst = cnt + nb;                          // nb: first staged position of each bucket
-    uint2* stage = (uint2*)(lst + nb + (nb & 1u));     // kLogSeg records, 8-byte aligned
+    uint2* stage = (uint2*)(lst + nb + (nb & 1u));     // kTileRecs records, 8-byte aligned
     __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t >> 6;
     for (uint32_t b = t; b < nb; b += 1024) gofs[b] = off[(uint64_t)b * gridDim.x + blockIdx.x];
     const uint32_t per = (nb + 1023) / 1024;           // counters per lane in the prefix sum
     const uint32_t b0 = min(t * per, nb), b1 = min(b0 + per, nb);
     const uint32_t nseg = log_segments(next, segs);
-    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-        const uint32_t n = min(fill[sg], kLogSeg);
-        const uint2* r = log + (uint64_t)sg * kLogSeg;
+    for (uint32_t tile = blockIdx.x; tile * kTileSegs < nseg; tile += gridDim.x) {
+        // the tile's records: segment s0's fill first, then the next segments'
+        uint32_t nfill[kTileSegs];
+        uint32_t n = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kTileSegs; j++) {
+            const uint32_t sg = tile * kTileSegs + j;
+            nfill[j] = sg < nseg ? min(fill[sg], kLogSeg) : 0u;
+            n += nfill[j];
+        }
+        const uint2* r = log + (uint64_t)tile * kTileRecs;
         for (uint32_t b = t; b < nb; b += 1024) cnt[b] = 0u;
         __syncthreads();
-        uint2 v[4];
-        uint32_t rk[4];
+        uint2 v[kPerLane];
+        uint32_t rk[kPerLane];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint32_t i = t + k * 1024;
+        for (uint32_t k = 0; k < kPerLane; k++) {
+            const uint32_t i = t + k * 1024;            // record i of the tile
+            uint32_t seg = 0, at = i;
+#pragma unroll
+            for (uint32_t j = 0; j + 1 < kTileSegs; j++)
+                if (seg == j && at >= nfill[j]) { at -= nfill[j]; seg = j + 1; }
             v[k] = make_uint2(0u, 0u);
             rk[k] = 0u;
             if (i < n) {
-                v[k] = r[i];
+                v[k] = r[(uint64_t)seg * kLogSeg + at];
                 rk[k] = atomicAdd(&cnt[v[k].x >> kBucketBits], 1u);
             }
         }
@@ -824,28 +842,33 @@ __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, con
         uint32_t x = sum;                              // inclusive scan over the wave
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(x, o, 64);
-            if ((int)__lane_id() >= o) x += y;
+            if ((int)lane >= o) x += y;
         }
-        if (__lane_id() == 63) wsum[t >> 6] = x;
+        if (lane == 63) wsum[wave] = x;
         __syncthreads();
-        uint32_t run = x - sum;
-        for (uint32_t w = 0; w < (t >> 6); w++) run += wsum[w];
+        uint32_t w = lane < 16 ? wsum[lane] : 0u;      // the waves' totals, scanned in every wave
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t y = __shfl_up(w, o, 64);
+            if ((int)lane >= o) w += y;
+        }
+        const uint32_t before = wave ? (uint32_t)__shfl((int)w, (int)wave - 1, 64) : 0u;
+        uint32_t run = before + x - sum;
         for (uint32_t b = b0; b < b1; b++) {
             lst[b] = run;
             run += cnt[b];
         }
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+        for (uint32_t k = 0; k < kPerLane; k++)
             if (t + k * 1024 < n) stage[lst[v[k].x >> kBucketBits] + rk[k]] = v[k];
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
+        for (uint32_t k = 0; k < kPerLane; k++) {
             const uint32_t i = t + k * 1024;
             if (i < n) {
-                const uint2 w = stage[i];
-                const uint32_t b = w.x >> kBucketBits;
-                out[gofs[b] + (i - lst[b])] = w;
+                const uint2 q = stage[i];
+                const uint32_t b = q.x >> kBucketBits;
+                out[gofs[b] + (i - lst[b])] = q;
             }
         }
         __syncthreads();
@@ -1320,7 +1343,7 @@ void Session::enqueue_log_passes(void* stream) {
     hipLaunchKernelGGL(dev::log_hist_kernel, dim3(g), dim3(1024), hist_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
                        d_dlog_next_, segs, nb, d_dlog_hist_);
     hipLaunchKernelGGL(dev::log_scan_kernel, dim3(1), dim3(1024), 0, s, d_dlog_hist_, nb, (uint32_t)g, d_dlog_start_);
-    const size_t scatter_lds = ((size_t)3 * nb + (nb & 1u)) * 4 + (size_t)kLogSeg * 8;
+    const size_t scatter_lds = ((size_t)3 * nb + (nb & 1u)) * 4 + (size_t)kTileSegs * kLogSeg * 8;
     hipLaunchKernelGGL(dev::log_scatter_kernel, dim3(g), dim3(1024), scatter_lds, s, (const uint2*)d_dlog_, d_dlog_fill_,
                        d_dlog_next_, segs, nb, d_dlog_hist_, (uint2*)d_dlog_sorted_);
     // several workgroups per bucket when there are fewer buckets than four per CU
