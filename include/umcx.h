@@ -86,6 +86,9 @@ typedef struct umcx_info {
     double deposit_cache_hits;        /* share of the pilot's global deposits that hit that cache */
     int32_t deposit_log;              /* 1 when runs append global deposits to the deposit log */
     double global_deposits_per_photon; /* the pilot's estimate for the runs (0 before a pilot) */
+    int32_t vote;                     /* scatter-round lane threshold the runs use (0: none); with
+                                         umcx_launch.vote -1 the pilot picks 16 when it counts fewer
+                                         than 0.3 scatters per voxel crossing, else 0 */
 } umcx_info;
 
 /* Build a session from a complete umcx JSON config (the MCX_userio input, src/umcx.cpp:451-632)

@@ -4,7 +4,8 @@ atomics. Only the float summation order of a cell changes, so every bar is the a
 fluence within 1e-4 relative L2 of the reference's volume, energies and detected rows unchanged,
 per-photon states bitwise. Covered: forced on for golden cases with labels in LDS and in HBM,
 with and without the per-lane deposit cache and the LDS tally box; runs split into chunks by a
-small log; a log too small for its chunk (deposits past it go to atomics); the pilot's choice."""
+small log; a log too small for its chunk (deposits past it go to atomics); the pilot's choice of the log and of the
+scatter-round vote."""
 import copy
 import sys
 
@@ -117,6 +118,7 @@ def test_pilot_chooses_the_log_where_atomics_would_bind():
         sim.run()
         assert sim.deposit_log()
         assert sim.global_deposits_per_photon() > 300
+        assert sim.vote() == 16          # 0.11 scatters per crossing: scatter rounds wait for 16 lanes
         flu_l = sim.fluence()
         e_l = sim.energies()
     with U.Simulation(skin, deposit_log=-1) as sim:
@@ -133,3 +135,4 @@ def test_pilot_chooses_the_log_where_atomics_would_bind():
         sim.run()
         assert not sim.deposit_log()
         assert 50 < sim.global_deposits_per_photon() < 200
+        assert sim.vote() == 0           # 0.83 scatters per crossing: scatter right away

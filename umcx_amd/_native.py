@@ -29,7 +29,7 @@ class UmcxInfo(C.Structure):
                 ("tally_mode", C.c_int32), ("label_bytes", C.c_int32), ("lds_label_bits", C.c_int32),
                 ("last_block", C.c_int32), ("last_blocks", C.c_int32), ("deposit_cache", C.c_int32),
                 ("deposit_cache_hits", C.c_double), ("deposit_log", C.c_int32),
-                ("global_deposits_per_photon", C.c_double)]
+                ("global_deposits_per_photon", C.c_double), ("vote", C.c_int32)]
 
 
 class UmcxBatchTrial(C.Structure):

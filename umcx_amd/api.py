@@ -127,6 +127,13 @@ class Simulation:
         check(lib().umcx_session_info(self._h, C.byref(info)))
         return float(info.global_deposits_per_photon)
 
+    def vote(self) -> int:
+        """Lanes of a wave that must wait to scatter before a scatter round runs (0: none), as the
+        runs use it now (the pilot picks it unless the session was given one)."""
+        info = UmcxInfo()
+        check(lib().umcx_session_info(self._h, C.byref(info)))
+        return int(info.vote)
+
     def deposit_cache_hits(self) -> float:
         """Share of the pilot's global deposits that hit the per-lane deposit cache (0 before a pilot)."""
         info = UmcxInfo()

@@ -224,6 +224,7 @@ int umcx_session_info(const umcx_session* s, umcx_info* info) {
         info->deposit_cache_hits = ss.deposit_cache_hits();
         info->deposit_log = ss.deposit_log() ? 1 : 0;
         info->global_deposits_per_photon = ss.global_deposits_per_photon();
+        info->vote = ss.vote();
         return 0;
     });
 }

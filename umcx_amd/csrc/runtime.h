@@ -5,6 +5,7 @@
 // the MCX_kernel loop (src/umcx.cpp:752-775) with the hand-written gfx950 kernel. Tallies
 // accumulate across runs until reset(), so a photon range can be split over calls.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -92,6 +93,7 @@ public:
     // float atomics: chosen by the pilot when they would saturate the memory-side atomics
     bool deposit_log() const;
     double global_deposits_per_photon() const { return dlog_rpp_; }   // pilot estimate (0 before one)
+    int vote() const { return lc_.vote < 0 ? vote_auto_ : std::min(lc_.vote, 64); }   // scatter-round threshold
     void* stream() const { return stream_; }
     // device pointers, for callers that keep results on the GPU
     float* device_fluence() const { return d_fluence_; }
@@ -139,6 +141,7 @@ private:
     size_t per_wg_lds_ = 0;     // LDS per workgroup at the session's launch shape
     // deposit log: records {cell, value}, the same records grouped by bucket, each segment's
     // fill, the segment counter, the (bucket, workgroup) counts and the bucket starts
+    int vote_auto_ = 0;                  // scatter-round lane vote chosen by the pilot (0: none)
     bool dlog_auto_ = false;             // the pilot chose the log
     bool dlog_unavailable_ = false;      // the device could not hold a log: float atomics
     double dlog_rpp_ = 0.0;              // global deposits per photon expected from the pilot
