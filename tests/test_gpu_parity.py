@@ -127,9 +127,10 @@ def test_detected_rows_match_reference(name, label_cache):
 
 
 def test_label_staging_choice():
-    """Volumes whose media ids pack into <= 96 KiB are staged in LDS (2 bits for <= 4 media, 4 bits
-    for <= 16); larger ones, and label_cache -1, read labels from HBM."""
-    expect = {"c2": 2, "c4src": 2, "cube60": 2, "zipvol": 2, "spherebox": 2, "sphshells": 0, "c3": 0}
+    """Volumes whose media ids pack into <= 96 KiB are staged in LDS (2 bits for a run of at most 4
+    ids, 4 bits for 16: sphshells uses ids 1-4 of its 5 media, 2 bits); larger ones, and
+    label_cache -1, are not."""
+    expect = {"c2": 2, "c4src": 2, "cube60": 2, "zipvol": 2, "spherebox": 2, "sphshells": 2, "c3": 0}
     for name, bits in expect.items():
         g = O.golden(name)
         with U.Simulation(g["cfg"]) as sim:

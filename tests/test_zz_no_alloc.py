@@ -148,7 +148,7 @@ def test_noalloc_run_equals_session_run_on_two_calls():
     a2 = r2.results()
     say(f"noalloc call 2 done: {a2['detected']} detections; session run")
     b2 = session_results(small, 50000, True)
-    assert b2["lds_bits"] == 4
+    assert b2["lds_bits"] == 2          # the session packs ids 1-4 as 2-bit codes; this entry point, the ids at 4 bits
     check_same(a2, b2)
     assert a2["detected"] > 0 and a2["rows"].shape[1] == 1 + 5 + 6
 

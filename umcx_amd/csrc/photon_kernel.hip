@@ -110,7 +110,7 @@ struct KernelArgs {
     uint64_t gstride;
     float unit;              // mm per voxel
     float* rw;               // final weight of each replayed photon (may be null)
-    // media ids packed plabel_bits (1, 2 or 4) per voxel, staged in LDS (LDSLAB kernels); code c
+    // media ids packed plabel_bits (1, 2 or 4) per voxel, staged in LDS (LAB 1 kernels); code c
     // stands for medium plabel_id0 + c * plabel_step
     const uint32_t* plabels;
     uint32_t plabel_words, plabel_lg, plabel_bits, plabel_id0, plabel_step;
@@ -177,13 +177,15 @@ __device__ __forceinline__ void log_deposit(volatile uint32_t* s_seg, volatile u
     else unsafeAtomicAdd(a->fluence + cell, val);
 }
 
-template <bool LDSLAB, typename LabelT>
+// Label modes (LAB): 0 labels read from the label volume in HBM; 1 media ids packed in LDS.
+constexpr int kLabHbm = 0, kLabLds = 1;
+template <int LAB, typename LabelT>
 struct LabelPick {
     using T = GlobalLabels<LabelT>;
     static __device__ __forceinline__ T make(const KernelArgs& a, const uint32_t*) { return T{(const LabelT*)a.labels}; }
 };
 template <typename LabelT>
-struct LabelPick<true, LabelT> {
+struct LabelPick<kLabLds, LabelT> {
     using T = LdsLabels;
     static __device__ __forceinline__ T make(const KernelArgs& a, const uint32_t* s) {
         return T{s, a.plabel_lg, a.plabel_bits, a.plabel_id0, a.plabel_step};
@@ -212,9 +214,9 @@ __device__ __forceinline__ void set_props(Photon& p, const KernelArgs& a, const 
 // REPLAY: re-trace the photons listed in a.rids (photon i is a pure function of i) and add
 // each voxel segment's perturbation-MC terms to the gradient slices instead of tallying.
 // DLOG: global deposits go to the deposit log (a.dlog) instead of float atomics.
-template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY, bool LDSLAB, bool DLOG = false>
+template <bool REFLECT, bool SAVEDET, typename LabelT, bool REPLAY, int LAB, bool DLOG = false>
 __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
-    // LDS: libm tables | media | packed labels (LDSLAB) | tally tile | per-lane partial paths
+    // LDS: libm tables | media | packed labels (LAB 1) | tally tile | per-lane partial paths
     extern __shared__ float4 lds4[];
     __shared__ uint32_t s_taken;                                 // static photon slots handed out
     __shared__ uint32_t s_lseg[16], s_lfill[16];                 // deposit log: each wave's segment and fill
@@ -223,7 +225,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     uint64_t* sexp = (uint64_t*)(lds4 + 16);                     // 32 x 8 B
     float4* smedia = lds4 + 32;
     uint32_t* slab = (uint32_t*)(smedia + a.mediumnum);
-    float* stile = (float*)(slab + (LDSLAB ? a.plabel_words : 0u));
+    float* stile = (float*)(slab + (LAB == kLabLds ? a.plabel_words : 0u));
     float* sppath = stile + a.tile_n;
     const int tid = threadIdx.x, bd = blockDim.x;
     if (tid < 16) slog[tid] = libm::kLogfTab[tid];
@@ -236,7 +238,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     const Tabs tabs{slog, sexp};
     const Grid grid{a.nx, a.ny, a.nz, a.nyz};
     for (int i = tid; i < a.mediumnum; i += bd) smedia[i] = a.media[i];
-    if (LDSLAB)
+    if (LAB == kLabLds)
         for (uint32_t i = tid; i < a.plabel_words; i += bd) slab[i] = a.plabels[i];
     for (int i = tid; i < (int)a.tile_n; i += bd) stile[i] = 0.f;
     // partial path lengths per medium: in registers for up to kRegPaths media, else in LDS
@@ -244,10 +246,10 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     if (SAVEDET && !regpp)
         for (int i = tid; i < a.ppathlen * bd; i += bd) sppath[i] = 0.f;
     __syncthreads();
-    const auto labs = LabelPick<LDSLAB, LabelT>::make(a, slab);
+    const auto labs = LabelPick<LAB, LabelT>::make(a, slab);
     // detector flag of the voxel a photon ends in (LDS labels carry media ids only)
     auto det_flag = [&](const Photon& q) -> uint32_t {
-        if (LDSLAB) return load_label<LabelT>(a.labels, q.lastvox) & kDet;
+        if (LAB != kLabHbm) return load_label<LabelT>(a.labels, q.lastvox) & kDet;
         return q.med & kDet;
     };
 
@@ -927,52 +929,52 @@ namespace {
 
 using LaunchFn = void (*)(const dev::KernelArgs&, int, int, size_t, hipStream_t);
 
-template <bool R, bool D, typename L, bool P = false, bool S = false, bool G = false>
+template <bool R, bool D, typename L, bool P = false, int S = dev::kLabHbm, bool G = false>
 void launch_one(const dev::KernelArgs& ka, int blocks, int block, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((dev::photon_kernel<R, D, L, P, S, G>), dim3(blocks), dim3(block), lds, s, ka);
 }
 
-// kernel variants: reflection, detectors, label type, replay, labels in LDS (uint8 labels only),
-// deposit log (G)
-template <bool P, bool G = false>
-LaunchFn pick_variant(bool reflect, bool savedet, bool u8, bool lds) {
-    if (u8 && lds) {
+// kernel variants: reflection, detectors, label type, replay, label mode (LDS-staged ids with
+// uint8 labels only), deposit log (G)
+template <bool P, bool G, int S>
+LaunchFn pick_mode(bool reflect, bool savedet, bool u8) {
+    if (u8 || S != dev::kLabHbm) {
         if (reflect)
-            return savedet ? launch_one<true, true, uint8_t, P, true, G> : launch_one<true, false, uint8_t, P, true, G>;
-        return savedet ? launch_one<false, true, uint8_t, P, true, G> : launch_one<false, false, uint8_t, P, true, G>;
-    }
-    if (u8) {
-        if (reflect)
-            return savedet ? launch_one<true, true, uint8_t, P, false, G> : launch_one<true, false, uint8_t, P, false, G>;
-        return savedet ? launch_one<false, true, uint8_t, P, false, G> : launch_one<false, false, uint8_t, P, false, G>;
+            return savedet ? launch_one<true, true, uint8_t, P, S, G> : launch_one<true, false, uint8_t, P, S, G>;
+        return savedet ? launch_one<false, true, uint8_t, P, S, G> : launch_one<false, false, uint8_t, P, S, G>;
     }
     if (reflect)
-        return savedet ? launch_one<true, true, uint32_t, P, false, G> : launch_one<true, false, uint32_t, P, false, G>;
-    return savedet ? launch_one<false, true, uint32_t, P, false, G> : launch_one<false, false, uint32_t, P, false, G>;
+        return savedet ? launch_one<true, true, uint32_t, P, S, G> : launch_one<true, false, uint32_t, P, S, G>;
+    return savedet ? launch_one<false, true, uint32_t, P, S, G> : launch_one<false, false, uint32_t, P, S, G>;
+}
+template <bool P, bool G = false>
+LaunchFn pick_variant(bool reflect, bool savedet, bool u8, int lab) {
+    if (lab == dev::kLabLds) return pick_mode<P, G, dev::kLabLds>(reflect, savedet, u8);
+    return pick_mode<P, G, dev::kLabHbm>(reflect, savedet, u8);
 }
 
-LaunchFn pick_replay(bool reflect, bool u8, bool lds) { return pick_variant<true>(reflect, false, u8, lds); }
+LaunchFn pick_replay(bool reflect, bool u8, int lab) { return pick_variant<true>(reflect, false, u8, lab); }
 
-LaunchFn pick(bool reflect, bool savedet, bool u8, bool lds, bool dlog = false) {
-    return dlog ? pick_variant<false, true>(reflect, savedet, u8, lds) : pick_variant<false>(reflect, savedet, u8, lds);
+LaunchFn pick(bool reflect, bool savedet, bool u8, int lab, bool dlog = false) {
+    return dlog ? pick_variant<false, true>(reflect, savedet, u8, lab) : pick_variant<false>(reflect, savedet, u8, lab);
 }
 
-template <bool R, bool D, typename L, bool S>
+template <bool R, bool D, typename L, int S>
 const void* kptr() {
     return (const void*)dev::photon_kernel<R, D, L, false, S>;
 }
-
-const void* kernel_ptr(bool reflect, bool savedet, bool u8, bool lds) {
-    if (u8 && lds) {
-        if (reflect) return savedet ? kptr<true, true, uint8_t, true>() : kptr<true, false, uint8_t, true>();
-        return savedet ? kptr<false, true, uint8_t, true>() : kptr<false, false, uint8_t, true>();
+template <int S>
+const void* kptr_mode(bool reflect, bool savedet, bool u8) {
+    if (u8 || S != dev::kLabHbm) {
+        if (reflect) return savedet ? kptr<true, true, uint8_t, S>() : kptr<true, false, uint8_t, S>();
+        return savedet ? kptr<false, true, uint8_t, S>() : kptr<false, false, uint8_t, S>();
     }
-    if (u8) {
-        if (reflect) return savedet ? kptr<true, true, uint8_t, false>() : kptr<true, false, uint8_t, false>();
-        return savedet ? kptr<false, true, uint8_t, false>() : kptr<false, false, uint8_t, false>();
-    }
-    if (reflect) return savedet ? kptr<true, true, uint32_t, false>() : kptr<true, false, uint32_t, false>();
-    return savedet ? kptr<false, true, uint32_t, false>() : kptr<false, false, uint32_t, false>();
+    if (reflect) return savedet ? kptr<true, true, uint32_t, S>() : kptr<true, false, uint32_t, S>();
+    return savedet ? kptr<false, true, uint32_t, S>() : kptr<false, false, uint32_t, S>();
+}
+const void* kernel_ptr(bool reflect, bool savedet, bool u8, int lab) {
+    if (lab == dev::kLabLds) return kptr_mode<dev::kLabLds>(reflect, savedet, u8);
+    return kptr_mode<dev::kLabHbm>(reflect, savedet, u8);
 }
 
 template <typename T>
@@ -1035,14 +1037,34 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
         }
         if (nids == 2 && ids[1] < ids[0]) std::swap(ids[0], ids[1]);
         if (nids == 1) ids[1] = ids[0];
-        auto pack = [&](uint32_t bits, uint64_t* words_out) -> uint32_t* {
+        // The media ids the volume uses: when they form an arithmetic run id0, id0 + step, ...
+        // of at most 16, the codes (id - id0) / step take 2 bits (4 ids) or 4 bits (16 ids)
+        // whatever the ids themselves are (the 200^3 skin model uses ids 1-4 of 5 media: 2 bits);
+        // otherwise the ids are packed as they are, for volumes of at most 16 media.
+        std::vector<uint8_t> seen(128, 0);
+        for (uint64_t i = 0; i < nvox; i++) seen[(uint32_t)p.labels[i] & 0x7Fu] = 1;
+        std::vector<uint32_t> used;
+        for (uint32_t m = 0; m < 128; m++)
+            if (seen[m]) used.push_back(m);
+        uint32_t cid0 = 0, cstep = 1, wbits = p.mediumnum <= 4 ? 2u : (p.mediumnum <= 16 ? 4u : 0u);
+        if (!used.empty() && used.size() <= 16) {
+            const uint32_t d = used.size() > 1 ? used[1] - used[0] : 1u;
+            bool run = true;
+            for (size_t k = 0; k < used.size(); k++) run = run && used[k] == used[0] + (uint32_t)k * d;
+            if (run) {
+                cid0 = used[0];
+                cstep = d;
+                wbits = used.size() <= 4 ? 2u : 4u;
+            }
+        }
+        auto pack = [&](uint32_t bits, uint32_t id0, uint32_t step, uint64_t* words_out,
+                        uint64_t max_bytes) -> uint32_t* {
             const uint64_t words = (nvox * bits + 31) / 32;
-            if (lc.label_cache < 0 || words * 4 > kLabelCacheBytes) return nullptr;
+            if (lc.label_cache < 0 || words * 4 > max_bytes) return nullptr;
             std::vector<uint32_t> packed(words, 0u);
             const uint32_t per = 32 / bits;
             for (uint64_t i = 0; i < nvox; i++) {
-                uint32_t code = (uint32_t)p.labels[i] & kMedMask;
-                if (bits == 1) code = code == ids[0] ? 0u : 1u;
+                const uint32_t code = (((uint32_t)p.labels[i] & kMedMask) - id0) / step;
                 packed[i / per] |= code << ((i % per) * bits);
             }
             uint32_t* d = dalloc<uint32_t>(words);
@@ -1050,25 +1072,27 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
             *words_out = words;
             return d;
         };
-        const uint32_t wbits = p.mediumnum <= 4 ? 2u : (p.mediumnum <= 16 ? 4u : 0u);
         uint64_t words = 0;
-        if (wbits && (d_plabels_ = pack(wbits, &words))) {
+        if (wbits && (d_plabels_ = pack(wbits, cid0, cstep, &words, kLabelCacheBytes))) {
             plabel_words_ = (uint32_t)words;
             plabel_bits_ = wbits;
+            plabel_id0_ = cid0;
+            plabel_step_ = cstep;
         }
         if (nids <= 2) {
-            uint32_t* d1 = pack(1, &words);
+            const uint32_t step1 = ids[1] > ids[0] ? ids[1] - ids[0] : 1u;
+            uint32_t* d1 = pack(1, ids[0], step1, &words, kLabelCacheBytes);
             if (d1 && !d_plabels_) {                 // the 1-bit form is the only one that fits
                 d_plabels_ = d1;
                 plabel_words_ = (uint32_t)words;
                 plabel_bits_ = 1;
                 plabel_id0_ = ids[0];
-                plabel_step_ = ids[1] - ids[0];
+                plabel_step_ = step1;
             } else if (d1) {
                 d_plabels1_ = d1;
                 plabel1_words_ = (uint32_t)words;
                 plabel1_id0_ = ids[0];
-                plabel1_step_ = ids[1] - ids[0];
+                plabel1_step_ = step1;
             }
         }
     } else {
@@ -1105,7 +1129,7 @@ Session::Session(const Problem& p, const LaunchConfig& lc) : prob_(p), lc_(lc) {
     if (lds > lds_wg_max) throw std::runtime_error("too many media for the LDS staging of media and partial paths");
     int per_cu = 0;
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0), block_, lds));
+    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel_ptr(reflect, savedet, label_bytes_ == 1, label_mode()), block_, lds));
     if (per_cu < 1) per_cu = 1;
     // labels in HBM: at most kHbmLabelWgPerCu workgroups per CU, so each keeps a 40 KiB LDS share
     // for its tally tile (200^3 skin model: 2256 ms at 4 per CU, 2304 ms at the 5 registers allow)
@@ -1239,7 +1263,7 @@ void Session::choose_tile(uint64_t first, uint64_t count) {
     ka.dep_cache = cache ? 1 : 0; ka.dep_stats = cache ? d_c + 2 : nullptr;
     ka.tile_n = 0; ka.tsx = ka.tsy = ka.tsz = ka.tsg = 0;
     const bool reflect = p.isreflect != 0;
-    pick(reflect, false, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
+    pick(reflect, false, label_bytes_ == 1, label_mode())(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
     enqueue_box_search(d_cnt, n, p.maxgate, (int)p.nx, (int)p.ny, (int)p.nz, tile_budget_, d_c + 8, s);
     unsigned long long head[16];
@@ -1411,9 +1435,9 @@ RunStats Session::run(uint64_t first, uint64_t count) {
     last_blocks_ = rblocks;
     const bool reflect = p.isreflect != 0, savedet = p.issavedet != 0;
     size_t lds = lds_bytes(ka.tile_n);
-    const LaunchFn launch = pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0);
+    const LaunchFn launch = pick(reflect, savedet, label_bytes_ == 1, label_mode());
     if (deposit_log() && log_ready(count)) {
-        const LaunchFn launch_log = pick(reflect, savedet, label_bytes_ == 1, plabel_words_ > 0, true);
+        const LaunchFn launch_log = pick(reflect, savedet, label_bytes_ == 1, label_mode(), true);
         // global deposits go to the log; a run larger than the log holds runs in chunks, each
         // launch followed by the passes that add its records to the grid
         const double rpp = dlog_rpp_ > 0.0 ? dlog_rpp_ : 512.0;
@@ -1553,7 +1577,7 @@ RunStats Session::replay(const uint32_t* ids, const int32_t* slices, const float
     hipLaunchKernelGGL(dev::zero_u64, dim3(1), dim3(64), 0, s, d_cursor_, 2);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev0_, s));
-    pick_replay(prob_.isreflect != 0, label_bytes_ == 1, plabel_words_ > 0)(ka, blocks_, block_, lds_bytes(0, false), s);
+    pick_replay(prob_.isreflect != 0, label_bytes_ == 1, label_mode())(ka, blocks_, block_, lds_bytes(0, false), s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipEventRecord((hipEvent_t)ev1_, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -1764,7 +1788,7 @@ void run_device_noalloc(const umcx_device_problem* q, uint64_t first, uint64_t c
     hipLaunchKernelGGL(dev::pack_labels_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, q->labels, nvox,
                        bits, (uint32_t)words, plabels);
     HIP_OK(hipGetLastError());
-    pick(q->isreflect != 0, savedet, true, true)(ka, blocks, block, lds_of(block), s);
+    pick(q->isreflect != 0, savedet, true, dev::kLabLds)(ka, blocks, block, lds_of(block), s);
     HIP_OK(hipGetLastError());
 }
 

@@ -83,7 +83,7 @@ public:
     int last_block() const { return last_block_; }
     int last_blocks() const { return last_blocks_; }
     int tally_mode() const { return tile_[4] > 0 ? 1 : 0; }   // effective: 1 when an LDS tile is in use
-    int lds_label_bits() const { return (int)plabel_bits_; }  // 0 when labels are read from HBM
+    int lds_label_bits() const { return plabel_words_ ? (int)plabel_bits_ : 0; }   // 0: labels read from HBM
     // the cache keys are 32-bit fluence indices: grids of 2^32 - 1 cells or more never use it
     bool deposit_cache() const {
         return fluence_len_ < 0xFFFFFFFFull && (lc_.deposit_cache > 0 || (lc_.deposit_cache == 0 && dep_cache_));
@@ -134,6 +134,7 @@ private:
     uint64_t msrc_n_ = 0;                // photons per source (0: one source)
     uint64_t msrc_count_ = 0;            // sources in the table
     uint32_t* d_plabels_ = nullptr;      // packed media ids for LDS staging (null: not staged)
+    int label_mode() const { return plabel_words_ ? 1 : 0; }   // kernels' label mode: staged ids or bytes
     uint32_t plabel_words_ = 0, plabel_bits_ = 0, plabel_id0_ = 0, plabel_step_ = 1;
     // the 1-bit form of the staged labels, kept beside the 2-/4-bit one until the pilot picks
     uint32_t* d_plabels1_ = nullptr;
