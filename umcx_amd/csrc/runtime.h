@@ -155,7 +155,7 @@ private:
     uint64_t dlog_records_ = 0;          // capacity in records (a multiple of the segment size)
     void ensure_log(uint64_t count);
     bool log_ready(uint64_t count);
-    void enqueue_log_passes(void* stream);
+    void enqueue_log_passes(void* stream, uint64_t expect_records);
 
     void* build_args(uint64_t first, uint64_t count, void* out) const;
     size_t lds_bytes(uint32_t tile_n, bool paths_needed = true) const;   // dynamic LDS of a launch
