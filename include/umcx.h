@@ -59,7 +59,7 @@ typedef struct umcx_launch {
     int32_t deposit_log;   /* 0: global fluence deposits appended to a device log and summed into the
                               grid by separate passes when the pilot counts at least 200 of them per
                               photon (float atomics would saturate), 1: always, -1: never (float
-                              atomics); grids of 2^26 cells or more always use atomics */
+                              atomics); grids of 2^27 cells or more always use atomics */
     int32_t deposit_log_segments; /* log capacity in segments of 4096 records, 64 KiB of device memory
                                      each with the grouped copy (0: sized per run from the pilot's
                                      count, at most a quarter of the free device memory); a run that

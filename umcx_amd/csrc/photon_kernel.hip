@@ -59,8 +59,8 @@ constexpr uint32_t kLogSeg = 4096;                 // records per log segment (3
 constexpr uint32_t kLogNone = 0xFFFFFFFFu;         // a wave that holds no segment yet
 constexpr uint32_t kLogFull = 0xFFFFFFFEu;         // the log is full: the wave's deposits go to atomics
 constexpr uint32_t kTileSegs = 2;                 // log segments the scatter pass sorts together
-constexpr int kBucketBits = 14;                    // fluence cells per bucket of the reduce pass: 2^14 (64 KiB of LDS)
-constexpr uint32_t kMaxBuckets = 4096;             // buckets of the log passes' LDS counters (grids below 2^26 cells)
+constexpr int kBucketBits = 15;                    // fluence cells per bucket of the reduce pass: 2^15 (128 KiB of LDS)
+constexpr uint32_t kMaxBuckets = 4096;             // buckets of the log passes' LDS counters (grids below 2^27 cells)
 constexpr double kLogMinRecords = 200.0;           // global deposits per photon from which the log is used
 constexpr double kVoteBelowScatters = 0.3;         // pilot scatters per crossing below which scatter rounds wait ...
 constexpr int kVoteLanes = 16;                     // ... until this many lanes of a wave need one
@@ -818,8 +818,9 @@ __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, con
     const uint32_t per = (nb + 1023) / 1024;           // counters per lane in the prefix sum
     const uint32_t b0 = min(t * per, nb), b1 = min(b0 + per, nb);
     const uint32_t nseg = log_segments(next, segs);
-    for (uint32_t tile = blockIdx.x; tile * kTileSegs < nseg; tile += gridDim.x) {
-        // the tile's records: segment s0's fill first, then the next segments'
+    // the records of tile `tile`, record i = t + k * 1024 in v[k]: the first segment's fill first,
+    // then the next segments'
+    auto load_tile = [&](uint32_t tile, uint2 (&v)[kPerLane]) -> uint32_t {
         uint32_t nfill[kTileSegs];
         uint32_t n = 0;
 #pragma unroll
@@ -829,24 +830,30 @@ __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, con
             n += nfill[j];
         }
         const uint2* r = log + (uint64_t)tile * kTileRecs;
-        for (uint32_t b = t; b < nb; b += 1024) cnt[b] = 0u;
-        __syncthreads();
-        uint2 v[kPerLane];
-        uint32_t rk[kPerLane];
 #pragma unroll
         for (uint32_t k = 0; k < kPerLane; k++) {
-            const uint32_t i = t + k * 1024;            // record i of the tile
+            const uint32_t i = t + k * 1024;
             uint32_t seg = 0, at = i;
 #pragma unroll
             for (uint32_t j = 0; j + 1 < kTileSegs; j++)
                 if (seg == j && at >= nfill[j]) { at -= nfill[j]; seg = j + 1; }
-            v[k] = make_uint2(0u, 0u);
-            rk[k] = 0u;
-            if (i < n) {
-                v[k] = r[(uint64_t)seg * kLogSeg + at];
-                rk[k] = atomicAdd(&cnt[v[k].x >> kBucketBits], 1u);
-            }
+            v[k] = i < n ? r[(uint64_t)seg * kLogSeg + at] : make_uint2(0u, 0u);
         }
+        return n;
+    };
+    uint2 v[kPerLane];
+    uint32_t n = blockIdx.x * kTileSegs < nseg ? load_tile(blockIdx.x, v) : 0u;
+    for (uint32_t tile = blockIdx.x; tile * kTileSegs < nseg; tile += gridDim.x) {
+        for (uint32_t b = t; b < nb; b += 1024) cnt[b] = 0u;
+        __syncthreads();
+        uint32_t rk[kPerLane];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerLane; k++)
+            rk[k] = t + k * 1024 < n ? atomicAdd(&cnt[v[k].x >> kBucketBits], 1u) : 0u;
+        // the next tile's records load while this one is ranked, staged and written
+        uint2 vn[kPerLane];
+        const uint32_t nt = tile + gridDim.x;
+        const uint32_t nn = nt * kTileSegs < nseg ? load_tile(nt, vn) : 0u;
         __syncthreads();
         uint32_t sum = 0;
         for (uint32_t b = b0; b < b1; b++) sum += cnt[b];
@@ -884,6 +891,9 @@ __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, con
         }
         __syncthreads();
         for (uint32_t b = t; b < nb; b += 1024) gofs[b] += cnt[b];
+#pragma unroll
+        for (uint32_t k = 0; k < kPerLane; k++) v[k] = vn[k];
+        n = nn;
     }
 }
 
