@@ -59,6 +59,12 @@ constexpr uint32_t kLogSeg = 4096;                 // records per log segment (3
 constexpr uint32_t kLogNone = 0xFFFFFFFFu;         // a wave that holds no segment yet
 constexpr uint32_t kLogFull = 0xFFFFFFFEu;         // the log is full: the wave's deposits go to atomics
 constexpr uint32_t kTileSegs = 2;                 // log segments the scatter pass sorts together
+// Records a log holds at most: positions in the passes are 32-bit, and the reduce pass's
+// unrolled reads look up to 8 x 1024 records past a position. Each chunk of a run ends with its
+// slowest photons alone on the GPU (about 15 ms for the 200^3 skin model), so larger logs mean
+// fewer chunks: 2^32 records take 64 GiB with the grouped copy, a quarter of the MI355X's memory.
+constexpr uint64_t kLogMaxRecords = (1ull << 32) - (1ull << 20);
+static_assert(kLogMaxRecords + 8 * 1024 < (1ull << 32), "log positions stay 32-bit");
 constexpr int kBucketBits = 15;                    // fluence cells per bucket of the reduce pass: 2^15 (128 KiB of LDS)
 constexpr uint32_t kMaxBuckets = 4096;             // buckets of the log passes' LDS counters (grids below 2^27 cells)
 constexpr double kLogMinRecords = 200.0;           // global deposits per photon from which the log is used
@@ -1364,7 +1370,7 @@ bool Session::deposit_log() const {
 }
 
 // Log buffers for a run of `count` photons: as many records as the pilot expects (times 1.25),
-// at least four partly filled segments' worth per wave, at most 2^31 records and a quarter of the
+// at least four partly filled segments' worth per wave, at most kLogMaxRecords and a quarter of the
 // free device memory for the log and its grouped copy together; runs that need more are split
 // into chunks. An explicit capacity (LaunchConfig::deposit_log_segments) is taken as given.
 void Session::ensure_log(uint64_t count) {
@@ -1376,7 +1382,7 @@ void Session::ensure_log(uint64_t count) {
     } else {
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b));
-        const uint64_t cap = std::min<uint64_t>(1ull << 31, (uint64_t)(free_b + 16 * dlog_records_) / 4 / 16);
+        const uint64_t cap = std::min<uint64_t>(kLogMaxRecords, (uint64_t)(free_b + 16 * dlog_records_) / 4 / 16);
         want = std::max<uint64_t>((uint64_t)((double)count * rpp * 1.25), 4 * waves * kLogSeg);
         want = std::min(want, cap) / kLogSeg * kLogSeg;
         if (want < kLogSeg) throw std::runtime_error("no device memory left for the deposit log");
