@@ -158,8 +158,11 @@ __device__ __forceinline__ float4 ld4(const __attribute__((address_space(4))) fl
 // takes the next free segment from the device counter. Once the log is full the wave's deposits
 // go to the grid as atomics, so a run never loses one. s_seg/s_fill hold each wave's segment
 // and fill count in LDS, where every lane of the wave sees the latest values (volatile: the lane
-// that reads them may not be the one that wrote them last).
-__device__ __forceinline__ void log_deposit(volatile uint32_t* s_seg, volatile uint32_t* s_fill, uint32_t cell,
+// that reads them may not be the one that wrote them last). The pointers carry the LDS address
+// space: volatile accesses through generic pointers compile to flat loads and stores with
+// system-coherence bits, each followed by a wait for every vector memory access in flight.
+typedef __attribute__((address_space(3))) volatile uint32_t LdsVolatileU32;
+__device__ __forceinline__ void log_deposit(LdsVolatileU32* s_seg, LdsVolatileU32* s_fill, uint32_t cell,
                                             float val) {
     const ArgsRef a = reread_args();
     const unsigned long long act = __ballot(1);
@@ -283,7 +286,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     const bool energy_out = a.outputtype == kEnergy;
     // a global deposit: to the deposit log when the session keeps one, else a float atomic
     auto emit = [&](uint32_t cell, float v) {
-        if (DLOG) log_deposit(s_lseg, s_lfill, cell, v);
+        if (DLOG) log_deposit((LdsVolatileU32*)s_lseg, (LdsVolatileU32*)s_lfill, cell, v);
         else unsafeAtomicAdd(a.fluence + cell, v);
     };
 
@@ -415,7 +418,7 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
                             nmiss += cached_deposit(emit, dkey, dsum, (uint32_t)p.lastvox + (uint32_t)tshift * (uint32_t)a.nxyz, val);
                         } else {
                             const uint64_t cell = (uint64_t)p.lastvox + (uint64_t)tshift * a.nxyz;
-                            if (DLOG) log_deposit(s_lseg, s_lfill, (uint32_t)cell, val);   // cells < 2^32
+                            if (DLOG) log_deposit((LdsVolatileU32*)s_lseg, (LdsVolatileU32*)s_lfill, (uint32_t)cell, val);   // cells < 2^32
                             else unsafeAtomicAdd(a.fluence + cell, val);
                         }
                     }
@@ -504,8 +507,8 @@ __global__ void __launch_bounds__(1024) photon_kernel(KernelArgs a) {
     if (savevol) flush_deposits(emit, dkey, dsum);
     // the wave's last segment: how far it was filled
     if (DLOG && __lane_id() == 0) {
-        const uint32_t seg = ((volatile uint32_t*)s_lseg)[tid >> 6];
-        if (seg != kLogNone && seg != kLogFull) reread_args()->dlog_fill[seg] = ((volatile uint32_t*)s_lfill)[tid >> 6];
+        const uint32_t seg = ((LdsVolatileU32*)s_lseg)[tid >> 6];
+        if (seg != kLogNone && seg != kLogFull) reread_args()->dlog_fill[seg] = ((LdsVolatileU32*)s_lfill)[tid >> 6];
     }
     if (a.dep_stats) {
         const double g = wave_sum((double)nglob), m = wave_sum((double)nmiss), k = wave_sum((double)nscat);
