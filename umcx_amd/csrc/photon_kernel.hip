@@ -862,7 +862,13 @@ __global__ void __launch_bounds__(1024) log_scatter_kernel(const uint2* log, con
         // the next tile's records load while this one is ranked, staged and written
         uint2 vn[kPerLane];
         const uint32_t nt = tile + gridDim.x;
-        const uint32_t nn = nt * kTileSegs < nseg ? load_tile(nt, vn) : 0u;
+        uint32_t nn = 0;
+        if (nt * kTileSegs < nseg) {
+            nn = load_tile(nt, vn);
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kPerLane; k++) vn[k] = make_uint2(0u, 0u);
+        }
         __syncthreads();
         uint32_t sum = 0;
         for (uint32_t b = b0; b < b1; b++) sum += cnt[b];
@@ -917,7 +923,8 @@ __device__ __forceinline__ void lds_add_records(uint32_t* tile, const uint2 (&v)
     constexpr uint32_t kMask = (1u << kBucketBits) - 1u;
     uint32_t old[N], prev[N];
 #pragma unroll
-    for (int u = 0; u < N; u++) old[u] = tile[v[u].x & kMask];
+    for (int u = 0; u < N; u++)
+        old[u] = __hip_atomic_load(tile + (v[u].x & kMask), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
     for (int u = 0; u < N; u++)
         prev[u] = atomicCAS(tile + (v[u].x & kMask), old[u], __float_as_uint(__uint_as_float(old[u]) + __uint_as_float(v[u].y)));
