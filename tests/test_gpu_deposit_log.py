@@ -5,7 +5,7 @@ fluence within 1e-4 relative L2 of the reference's volume, energies and detected
 per-photon states bitwise. Covered: forced on for golden cases with labels in LDS and in HBM,
 with and without the per-lane deposit cache and the LDS tally box; runs split into chunks by a
 small log; a log too small for its chunk (deposits past it go to atomics); the pilot's choice of the log and of the
-scatter-round vote."""
+scatter-round vote; every deposit on a few hot cells (the reduce pass's contended adds)."""
 import copy
 import sys
 
@@ -136,3 +136,32 @@ def test_pilot_chooses_the_log_where_atomics_would_bind():
         assert not sim.deposit_log()
         assert 50 < sim.global_deposits_per_photon() < 200
         assert sim.vote() == 0           # 0.83 scatters per crossing: scatter right away
+
+
+def test_log_on_a_few_hot_cells():
+    """Every deposit lands in one of 4^3 cells (a 4 mm cube of strong absorption and scattering, no
+    LDS box): the reduce pass's compare-and-swap adds meet the same cells from many lanes at once,
+    and its equal record shares split the grid's one bucket between many workgroups, each adding
+    its share's sums to the same cells. The grid equals the atomics run's up to summation order
+    (float32 sums of ~1e5 deposits per cell: the fluence bar, 1e-4 relative L2) and holds the
+    deposited energy (energy output: each deposit is its weight drop)."""
+    cfg = {"Session": {"ID": "hot", "Photons": 200_000, "RNGSeed": 12345, "DoMismatch": 0,
+                       "DoSaveVolume": 1, "DoPartialPath": 0, "OutputType": "e"},
+           "Forward": {"T0": 0, "T1": 5e-9, "Dt": 5e-9},
+           "Domain": {"Dim": [4, 4, 4], "LengthUnit": 1,
+                      "Media": [{"mua": 0, "mus": 0, "g": 1, "n": 1}, {"mua": 0.5, "mus": 20, "g": 0.5, "n": 1}]},
+           "Optode": {"Source": {"Type": "pencil", "Pos": [2, 2, 0], "Dir": [0, 0, 1]}},
+           "Shapes": [{"Grid": {"Tag": 1, "Size": [4, 4, 4]}}]}
+    say("deposit log on 64 hot cells")
+    out = {}
+    for mode in (1, -1):
+        with U.Simulation(cfg, deposit_log=mode, tally_mode=0) as sim:
+            assert sim.deposit_log() == (mode == 1)
+            sim.run()
+            out[mode] = (sim.fluence().astype(np.float64), sim.energies())
+    (flu_l, e_l), (flu_a, e_a) = out[1], out[-1]
+    assert e_l[1] == e_a[1] and abs(e_l[2] - e_a[2]) <= 1e-9 * e_a[1]
+    assert e_l[2] > 0.3 * e_l[1]                       # about half the weight is deposited (oracle: 47 %)
+    assert rel_l2(flu_l, flu_a) <= 1e-4
+    assert float(flu_l.sum()) == pytest.approx(e_l[2], rel=1e-4)
+    assert float(flu_a.sum()) == pytest.approx(e_a[2], rel=1e-4)
