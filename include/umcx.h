@@ -258,10 +258,12 @@ int umcx_libm_host(int32_t fn, const float* in, float* out, uint64_t n);
 /* Same on the GPU (device ordinal), for bitwise checks of the device build. */
 int umcx_libm_device(int32_t fn, const float* in, float* out, uint64_t n, int32_t device);
 /* GPU self-check of the short correctly rounded sequences the photon loop uses away from special
- * operands (which 0: sqrt over every float in [2^-96, 2^128); 1: 1/x over every float with
- * 2^-60 <= |x| <= 2^60; 2: a/b on 2^32 hashed pairs of their domain) against the compiler's
- * full sequences; *mismatches counts differing results. No counterpart in the reference: it
- * backs the bitwise claim of the device arithmetic. */
+ * operands (which 0: sqrt over every float in [2^-96, 2^128); 1: the direction reciprocal over
+ * every float with 2^-60 <= |x| <= 2^60; 2: a/b on 2^32 hashed pairs of their domain; 3: the
+ * reciprocal over every normal float with |x| < 2^126; 4: a/b for every b in [1, 2) against
+ * eight edge mantissas of a) against the compiler's full sequences; *mismatches counts
+ * differing results. No counterpart in the reference: it backs the bitwise claim of the device
+ * arithmetic. */
 int umcx_arith_check_device(int32_t which, uint64_t* tested, uint64_t* mismatches, int32_t device);
 void umcx_free(void* p);
 

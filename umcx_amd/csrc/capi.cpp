@@ -526,7 +526,7 @@ int umcx_libm_device(int32_t fn, const float* in, float* out, uint64_t n, int32_
 
 int umcx_arith_check_device(int32_t which, uint64_t* tested, uint64_t* mismatches, int32_t device) {
     return guarded([&]() -> int {
-        if (which < 0 || which > 2 || !mismatches) throw std::runtime_error("umcx_arith_check_device: which must be 0, 1 or 2");
+        if (which < 0 || which > 4 || !mismatches) throw std::runtime_error("umcx_arith_check_device: which must be 0 to 4");
         *mismatches = umcx::device_arith_check(which, tested, device);
         return 0;
     });

@@ -219,15 +219,19 @@ UMCX_HD float sqrtf_cr(float x) {
 #endif
 }
 
-// Correctly rounded sqrt and division for operands away from the special ranges. On the GPU
-// these are the compiler's correctly rounded sequences without their operand scaling and
-// special-case fix-up, which change nothing in these ranges:
+// Correctly rounded sqrt, reciprocal and division for operands away from the special ranges,
+// shorter than the compiler's correctly rounded sequences, which scale their operands and fix up
+// special cases:
 //  * sqrt_ns(x): x finite and >= 2^-96 (the scaling threshold; zero, inf and NaN excluded): the
 //    hardware estimate, then the neighbour below or above when its residual says so;
+//  * rcp_ns(b): b normal with |b| < 2^126 (1/b normal): the hardware estimate refined by one
+//    Newton step (three instructions); the estimate alone is not correctly rounded, the refined
+//    value is on every float of that range (checked on the GPU);
 //  * div_ns(a, b): 2^-120 <= |b| <= 2^120, and a == +0 or (2^-100 <= |a| and 2^-120 <= |a/b| <=
-//    2^120) (no v_div_scale case applies and v_div_fixup passes the quotient through):
-//    reciprocal estimate, one Newton step, quotient and two residual corrections.
-// Bitwise equal to sqrtf_cr and a / b there (exhaustive and sampled GPU checks in the tests).
+//    2^120): q = a * rcp_ns(b), then one residual correction q + (a - b q) * rcp_ns(b), the
+//    residual exact by FMA. With the reciprocal correctly rounded this gives the correctly rounded
+//    quotient (Markstein's theorem; P. Markstein, IA-64 and Elementary Functions, 2000).
+// Bitwise equal to sqrtf_cr, 1 / b and a / b there (exhaustive and sampled GPU checks in the tests).
 UMCX_HD float sqrt_ns(float x) {
 #if defined(__HIPCC__)
     const float s = __builtin_amdgcn_sqrtf(x);
@@ -238,13 +242,19 @@ UMCX_HD float sqrt_ns(float x) {
     return sqrtf(x);
 #endif
 }
+UMCX_HD float rcp_ns(float b) {
+#if defined(__HIPCC__)
+    const float r = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+#else
+    return 1.0f / b;
+#endif
+}
 UMCX_HD float div_ns(float a, float b) {
 #if defined(__HIPCC__)
-    float r = __builtin_amdgcn_rcpf(b);
-    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
-    const float q0 = a * r;
-    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r, q0);
-    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), r, q1);
+    const float y = rcp_ns(b);
+    const float q = a * y;
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), y, q);
 #else
     return a / b;
 #endif
@@ -252,7 +262,7 @@ UMCX_HD float div_ns(float a, float b) {
 // 1 / x for a direction component (|x| <= 1 up to rounding): the short sequence for
 // 2^-60 <= |x| <= 2^60, the full correctly rounded division otherwise (zero, tiny or huge x).
 UMCX_HD float recip_dir(float x) {
-    return (fabsf(x) >= 0x1p-60f && fabsf(x) <= 0x1p60f) ? div_ns(1.0f, x) : 1.0f / x;
+    return (fabsf(x) >= 0x1p-60f && fabsf(x) <= 0x1p60f) ? rcp_ns(x) : 1.0f / x;
 }
 // ---- acosf: fdlibm single precision ----
 UMCX_HD float acos_rational(float z) {

@@ -135,7 +135,7 @@ __device__ __forceinline__ void rotate(Photon& p, float st, float ct, float sp, 
         ny = st * sp;
         nz = (p.vz > 0.f) ? ct : -ct;
     }
-    float s = libm::div_ns(1.f, libm::sqrt_ns(nx * nx + ny * ny + nz * nz));   // a near-unit vector
+    float s = libm::rcp_ns(libm::sqrt_ns(nx * nx + ny * ny + nz * nz));   // of a near-unit vector's norm
     p.vx = nx * s;
     p.vy = ny * s;
     p.vz = nz * s;
@@ -271,9 +271,16 @@ __device__ __forceinline__ void scatter(Photon& p, Rng& r, float4 prop, const Ta
     }
     libm::sincosf_host(th, &sth, &cth);
     rotate(p, sth, cth, sphi, cphi);
-    p.rx = libm::recip_dir(p.vx);
-    p.ry = libm::recip_dir(p.vy);
-    p.rz = libm::recip_dir(p.vz);
+    // one range test for the three components (a unit vector's components never exceed 2^60)
+    if (fminf(fminf(fabsf(p.vx), fabsf(p.vy)), fabsf(p.vz)) >= 0x1p-60f) {
+        p.rx = libm::rcp_ns(p.vx);
+        p.ry = libm::rcp_ns(p.vy);
+        p.rz = libm::rcp_ns(p.vz);
+    } else {
+        p.rx = libm::recip_dir(p.vx);
+        p.ry = libm::recip_dir(p.vy);
+        p.rz = libm::recip_dir(p.vz);
+    }
     p.vw += 1.f;
 }
 

@@ -682,6 +682,10 @@ __global__ void vprops_kernel(const float* mua, const float* mus, float unit, fl
 //   0: sqrt_ns(x) vs sqrtf_cr(x) for every float in [2^-96, 2^128)
 //   1: recip_dir(x) vs 1/x for every float with 2^-60 <= |x| <= 2^60, both signs
 //   2: div_ns(a, b) vs a/b on 2^32 hashed pairs inside div_ns's domain
+//   3: rcp_ns(x) vs 1/x for every normal float with |x| < 2^126, both signs
+//   4: div_ns(a, b) vs a/b for every b in [1, 2) against a of eight mantissas (all ones, all ones
+//      but the last bit, 1 ulp above a power of two, a power of two, the half-way mantissas and the
+//      alternating patterns) at exponents 0 and 1
 __device__ __forceinline__ uint32_t arith_hash(uint64_t k) {
     uint64_t z = k + 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -700,6 +704,16 @@ __global__ void arith_check_kernel(int which, uint64_t n, unsigned long long* ba
             const uint32_t mag = 0x21800000u + (uint32_t)(i % span);     // 2^-60 .. 2^60
             const float x = libm::u2f(mag | (i >= span ? 0x80000000u : 0u));
             same = libm::f2u(libm::recip_dir(x)) == libm::f2u(1.0f / x);
+        } else if (which == 3) {
+            const uint32_t span = (uint32_t)(n / 2);
+            const float x = libm::u2f((0x00800000u + (uint32_t)(i % span)) | (i >= span ? 0x80000000u : 0u));
+            same = libm::f2u(libm::rcp_ns(x)) == libm::f2u(1.0f / x);
+        } else if (which == 4) {
+            constexpr uint32_t kMant[8] = {0x7FFFFFu, 0x7FFFFEu, 0x000001u, 0x000000u,
+                                           0x400000u, 0x3FFFFFu, 0x555555u, 0x2AAAAAu};
+            const float b = libm::u2f((127u << 23) | (uint32_t)(i & 0x7FFFFFu));
+            const float a = libm::u2f(((127u + (uint32_t)(i >> 26)) << 23) | kMant[(i >> 23) & 7u]);
+            same = libm::f2u(libm::div_ns(a, b)) == libm::f2u(a / b);
         } else {
             // |b| in [2^-120, 2^120], |a/b| in [2^-120, 2^120], |a| >= 2^-100, random signs and mantissas
             const uint32_t h0 = arith_hash(2 * i), h1 = arith_hash(2 * i + 1);
@@ -1691,7 +1705,9 @@ void Session::fetch_trace(TraceRec* out, uint64_t n) const {
 uint64_t device_arith_check(int which, uint64_t* tested, int device) {
     HIP_OK(hipSetDevice(device));
     const uint64_t n = which == 0 ? (0x7F800000ull - 0x0F800000ull)
-                     : which == 1 ? 2ull * (0x5D800001ull - 0x21800000ull) : (1ull << 32);
+                     : which == 1 ? 2ull * (0x5D800001ull - 0x21800000ull)
+                     : which == 3 ? 2ull * (0x7E800000ull - 0x00800000ull)
+                     : which == 4 ? 2ull * 8ull * (1ull << 23) : (1ull << 32);
     unsigned long long* bad = dalloc<unsigned long long>(1);
     HIP_OK(hipMemset(bad, 0, 8));
     hipLaunchKernelGGL(dev::arith_check_kernel, dim3(8192), dim3(256), 0, 0, which, n, bad);
