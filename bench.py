@@ -372,7 +372,7 @@ def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3, target: fl
     sides scale with the photon count, so the ratio is expected to hold)."""
     import numpy as np
 
-    res = jacobian_at(sync, photons, fd_voxels)
+    res = jacobian_at(sync, photons, fd_voxels, warm_forward=True)
     worst = res["noise_rel_std_max"]
     n1 = int(np.ceil(photons * 1.1 * (worst / target) ** 2 / 1e6)) * 1_000_000 if worst > target else photons
     log(f"bench: jacobian at {n1:.3e} photons per source for <= {target:.0%} noise")
@@ -420,7 +420,7 @@ def inclusion_jacobian_noise(photons: int, batches: int = 8) -> dict:
             "photons_per_source_for_every_pair_at_1pct": int(np.ceil(photons * max(1.0, (rel.max() / 0.01) ** 2)))}
 
 
-def jacobian_at(sync, photons: int, fd_voxels: int) -> dict:
+def jacobian_at(sync, photons: int, fd_voxels: int, warm_forward: bool = False) -> dict:
     """d log I_sd / d(mua_v, mus_v) for 16 sources x 16 detectors on the 64x64x32 slab by one
     forward launch per source plus the replay kernel, against the loop of plain launches that
     central differences under common random numbers need (2 launches per (source, voxel,
@@ -439,6 +439,16 @@ def jacobian_at(sync, photons: int, fd_voxels: int) -> dict:
     w.jacobian()
     w.close()
     m = J.MultiSourceModel(cfg, srcs, photons)
+    first_wall = None
+    if warm_forward:
+        # PyTorch loads the sort and scan kernels it picks for this many records on their first
+        # use (about 80 ms, once per process), so the timed forward is the model's second; the
+        # first one's wall time is reported beside it
+        sync()
+        t0 = time.perf_counter()
+        m.forward()
+        sync()
+        first_wall = (time.perf_counter() - t0) * 1e3
     sync()
     t0 = time.perf_counter()
     m.forward()
@@ -486,6 +496,7 @@ def jacobian_at(sync, photons: int, fd_voxels: int) -> dict:
         "photons_per_source": photons, "entries": entries, "records": recs,
         "forward_ms": round(m.forward_ms, 2), "replay_ms": round(m.replay_ms, 2),
         "forward_wall_ms": round(fwd_wall, 2), "jacobian_wall_ms": round(jac_wall, 2),
+        "first_forward_wall_ms": None if first_wall is None else round(first_wall, 2),
         "entries_per_s": entries / (rep_ms * 1e-3),
         "fd_plain_launch_ms": round(per_launch_ms, 3), "fd_plain_launch_wall_ms": round(per_launch_wall, 3),
         "fd_loop_ms_full_jacobian": fd_total_ms,
