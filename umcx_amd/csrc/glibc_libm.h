@@ -14,9 +14,10 @@
 //   * acosf: the fdlibm single-precision algorithm (Sun Microsystems, 1993) that glibc's
 //     sysdeps/ieee754/flt-32 keeps: rational approximation of asin on [0, 0.5] and the
 //     half-angle reduction above it, in float.
-// Verified against the host libm on every float of logf (0,1], sinf/cosf [0,8), acosf [-1,1]
-// and expf [-104,88] (0 mismatches), and on the GPU over every argument the photon loop can form
-// (tests/test_host_contract.py, tests/test_gpu_parity.py).
+// Verified bit for bit against the host libm on the GPU over every positive normal float for
+// logf, every float for expf, sinf and cosf, and every float of [-1, 1] for acosf
+// (tests/test_gpu_libm_exhaustive.py), and on the CPU over every argument the photon loop can
+// form and sampled floats (tests/test_host_contract.py).
 //
 // Implemented ranges: logf for positive normal inputs (the loop's u + FLT_EPSILON >= 2^-23),
 // sinf/cosf for every argument (sincosf_host, the fast form for |x| < 120 used by scattering;
