@@ -388,7 +388,9 @@ def jacobian_leg(sync, photons: int = 10_000_000, fd_voxels: int = 3, target: fl
 def inclusion_jacobian_noise(photons: int, batches: int = 8) -> dict:
     """Noise of the Jacobian itself: d log I_sd / d mua summed over the inclusion's voxels, per
     source-detector pair, from `batches` independent seed batches of photons/batches per source
-    (the sample std across batches / sqrt(batches) is the relative std at `photons` per source)."""
+    (the sample std across batches / sqrt(batches) is the relative std at `photons` per source).
+    The same batches give the empirical relative std of the detected intensities I_sd at
+    `photons` per source, beside the single-run estimate sqrt(sum w^2) / sum w."""
     import copy
 
     import numpy as np
@@ -398,12 +400,13 @@ def inclusion_jacobian_noise(photons: int, batches: int = 8) -> dict:
     srcs = J.grid_sources(64, 64, 4, 16.5)
     lab = np.asarray(__import__("umcx_amd").domain(cfg) & 0x7FFFFFFF).reshape(-1)
     incl = np.flatnonzero(lab == 2)
-    vals = []
+    vals, ivals = [], []
     for b in range(batches):
         c = copy.deepcopy(cfg)
         c["Session"]["RNGSeed"] = 1648335518 + 7919 * (b + 1)
         m = J.MultiSourceModel(c, srcs, photons // batches)
         m.forward()
+        ivals.append(np.asarray(m.I, dtype=np.float64).reshape(-1))
         Jt = m.jacobian()
         import torch
         idx = torch.as_tensor(incl, device=Jt.device)
@@ -412,12 +415,18 @@ def inclusion_jacobian_noise(photons: int, batches: int = 8) -> dict:
         del Jt
     v = np.array(vals)                                         # (batches, pairs)
     rel = v.std(axis=0, ddof=1) / np.abs(v.mean(axis=0)) / np.sqrt(batches)
-    log(f"bench: inclusion Jacobian noise at {photons:.3e} per source: median {np.median(rel):.4f}, max {rel.max():.4f}")
+    iv = np.array(ivals)                                       # (batches, pairs)
+    irel = iv.std(axis=0, ddof=1) / iv.mean(axis=0) / np.sqrt(batches)
+    log(f"bench: inclusion Jacobian noise at {photons:.3e} per source: median {np.median(rel):.4f}, max {rel.max():.4f}; "
+        f"I_sd from the same batches: median {np.median(irel):.4f}, max {irel.max():.4f}")
     return {"quantity": "d log I_sd / d mua summed over the inclusion's 216 voxels, per pair",
             "batches": batches, "photons_per_source_per_batch": photons // batches,
             "noise_rel_std_median": float(np.median(rel)), "noise_rel_std_max": float(rel.max()),
             "pairs_at_or_below_1pct": int((rel <= 0.01).sum()), "pairs": int(rel.size),
-            "photons_per_source_for_every_pair_at_1pct": int(np.ceil(photons * max(1.0, (rel.max() / 0.01) ** 2)))}
+            "photons_per_source_for_every_pair_at_1pct": int(np.ceil(photons * max(1.0, (rel.max() / 0.01) ** 2))),
+            "intensity_noise_from_batches": {"quantity": "detected intensity I_sd per source-detector pair",
+                                             "noise_rel_std_median": float(np.median(irel)),
+                                             "noise_rel_std_max": float(irel.max())}}
 
 
 def jacobian_at(sync, photons: int, fd_voxels: int, warm_forward: bool = False) -> dict:
